@@ -1,0 +1,333 @@
+"""Python host mirror of the gfx950 AXHEL library.
+
+Thin ctypes binding of the C ABI in include/axhel_hip.h (libaxhel_hip.so),
+shaped like the reference's public API (unipi::axhel, axhel/include/axhel/*.hpp):
+the element-wise functions take (result, operand1, operand2/scalar, n, modulus,
+factor...) and NTT(degree, modulus, root_of_unity) has ComputeForward /
+ComputeInverse with the same factor conventions.  Buffers are torch uint64
+(or int64) tensors on the GPU, or numpy uint64 arrays on the host (staged).
+
+There is no CPU fallback: importing works without a GPU, but every compute
+call needs the built library and a GPU, and fails loudly otherwise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libaxhel_hip.so")
+CXX_LIB_PATH = os.path.join(LIB_DIR, "libaxhel.so")
+CXX_CHECK_PATH = os.path.join(LIB_DIR, "axhel_cxx_check")
+
+OK = 0
+UNSUPPORTED_FACTOR = 1
+INVALID_ARGUMENT = 2
+NO_MEMORY = 3
+HIP_ERROR = 4
+NO_DEVICE = 5
+REJECTED = 6
+TIMEOUT = 7
+CANCELLED = 8
+
+
+class AxhelError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        lib = _load()
+        msg = lib.axhel_hip_status_string(status).decode()
+        detail = lib.axhel_hip_last_error().decode()
+        super().__init__(f"{what}: {msg}" + (f" ({detail})" if detail else ""))
+
+
+_lib = None
+_u64 = ctypes.c_uint64
+_p = ctypes.c_void_p
+_st = ctypes.c_int
+
+
+class NttJob(ctypes.Structure):
+    """axhel_hip_ntt_job (include/axhel_hip.h)."""
+    _fields_ = [("plan", _p), ("result", _p), ("operand", _p), ("output_mod_factor", _u64),
+                ("inverse", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class NttConfig(ctypes.Structure):
+    """axhel_hip_ntt_config (include/axhel_hip.h)."""
+    _fields_ = [("log_elems", ctypes.c_int32), ("row_groups", ctypes.c_int32),
+                ("col_groups", ctypes.c_int32), ("log_col", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+# name -> (restype, argtypes); must cover every function declared in include/axhel_hip.h
+SIGNATURES = {
+    "axhel_hip_status_string": (ctypes.c_char_p, [_st]),
+    "axhel_hip_last_error": (ctypes.c_char_p, []),
+    "axhel_hip_device_count": (ctypes.c_int, []),
+    "axhel_hip_is_device_pointer": (ctypes.c_int, [_p]),
+    "axhel_hip_stream_synchronize": (_st, [_p]),
+    "axhel_hip_add_mod_vv": (_st, [_p, _p, _p, _u64, _u64, _p]),
+    "axhel_hip_add_mod_vs": (_st, [_p, _p, _u64, _u64, _u64, _p]),
+    "axhel_hip_sub_mod_vv": (_st, [_p, _p, _p, _u64, _u64, _p]),
+    "axhel_hip_sub_mod_vs": (_st, [_p, _p, _u64, _u64, _u64, _p]),
+    "axhel_hip_mul_mod": (_st, [_p, _p, _p, _u64, _u64, _u64, _p]),
+    "axhel_hip_fma_mod": (_st, [_p, _p, _u64, _p, _u64, _u64, _u64, _p]),
+    "axhel_hip_reduce_mod": (_st, [_p, _p, _u64, _u64, _u64, _u64, _p]),
+    "axhel_hip_ntt_plan_create": (_st, [_u64, _u64, _u64, ctypes.POINTER(_p)]),
+    "axhel_hip_ntt_plan_create_external": (_st, [_u64, _u64, _p, _p, _u64, _u64, ctypes.POINTER(_p)]),
+    "axhel_hip_ntt_plan_destroy": (_st, [_p]),
+    "axhel_hip_ntt_plan_degree": (_u64, [_p]),
+    "axhel_hip_ntt_plan_modulus": (_u64, [_p]),
+    "axhel_hip_ntt_plan_tables": (_st, [_p, _p, _p, _p]),
+    "axhel_hip_ntt_forward": (_st, [_p, _p, _p, _u64, _u64, _p]),
+    "axhel_hip_ntt_inverse": (_st, [_p, _p, _p, _u64, _u64, _p]),
+    "axhel_hip_ntt_batch_create": (_st, [ctypes.POINTER(NttJob), ctypes.c_size_t, ctypes.POINTER(_p)]),
+    "axhel_hip_ntt_batch_launch": (_st, [_p, _p]),
+    "axhel_hip_ntt_batch_launch_eager": (_st, [_p, _p]),
+    "axhel_hip_ntt_batch_destroy": (_st, [_p]),
+    "axhel_hip_ntt_batch_kernel_count": (ctypes.c_int, [_p]),
+    "axhel_hip_ntt_batch_launch_kernel": (_st, [_p, ctypes.c_int, _p]),
+    "axhel_hip_ntt_batch_run": (_st, [ctypes.POINTER(NttJob), ctypes.c_size_t, _p]),
+    "axhel_hip_ntt_config_count": (ctypes.c_int, [_u64]),
+    "axhel_hip_ntt_config_get": (_st, [_u64, ctypes.c_int, ctypes.POINTER(NttConfig)]),
+    "axhel_hip_ntt_config_select": (_st, [_u64, ctypes.c_int]),
+    "axhel_hip_ntt_config_selected": (ctypes.c_int, [_u64]),
+}
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m axhel_amd.build` "
+                          "(or __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def lib():
+    return _load()
+
+
+def _check(status, what):
+    if status != OK:
+        raise AxhelError(status, what)
+
+
+# ------------------------------------------------------------------ buffers
+def _ptr(x):
+    """Data pointer of a torch tensor (device or host) or numpy uint64 array."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        assert x.dtype == np.uint64 and x.flags["C_CONTIGUOUS"], "need contiguous uint64"
+        return x.ctypes.data
+    # torch tensor
+    assert x.is_contiguous(), "need a contiguous tensor"
+    assert x.element_size() == 8, "need 64-bit elements"
+    return x.data_ptr()
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream  # torch.cuda.Stream
+
+
+def device_count():
+    return int(_load().axhel_hip_device_count())
+
+
+def synchronize(stream=None):
+    _check(_load().axhel_hip_stream_synchronize(_stream_ptr(stream)), "synchronize")
+
+
+# ------------------------------------------------------------ element-wise
+def EltwiseAddMod(result, operand1, operand2, n, modulus, stream=None):
+    """eltwise-add-mod.hpp:13,17 -- operand2 is a buffer or a Python int."""
+    L = _load()
+    if isinstance(operand2, (int, np.integer)):
+        s = L.axhel_hip_add_mod_vs(_ptr(result), _ptr(operand1), int(operand2), n, modulus, _stream_ptr(stream))
+    else:
+        s = L.axhel_hip_add_mod_vv(_ptr(result), _ptr(operand1), _ptr(operand2), n, modulus, _stream_ptr(stream))
+    _check(s, "EltwiseAddMod")
+
+
+def EltwiseSubMod(result, operand1, operand2, n, modulus, stream=None):
+    """eltwise-sub-mod.hpp:13,17."""
+    L = _load()
+    if isinstance(operand2, (int, np.integer)):
+        s = L.axhel_hip_sub_mod_vs(_ptr(result), _ptr(operand1), int(operand2), n, modulus, _stream_ptr(stream))
+    else:
+        s = L.axhel_hip_sub_mod_vv(_ptr(result), _ptr(operand1), _ptr(operand2), n, modulus, _stream_ptr(stream))
+    _check(s, "EltwiseSubMod")
+
+
+def EltwiseMulMod(result, operand1, operand2, n, modulus, input_mod_factor, stream=None):
+    """eltwise-mul-mod.hpp:14.  Returns False (output untouched) for factors the
+    reference ignores, as its silent no-op."""
+    s = _load().axhel_hip_mul_mod(_ptr(result), _ptr(operand1), _ptr(operand2), n, modulus, input_mod_factor,
+                                  _stream_ptr(stream))
+    if s == UNSUPPORTED_FACTOR:
+        return False
+    _check(s, "EltwiseMulMod")
+    return True
+
+
+def EltwiseFMAMod(res, op1, op2, op3, n, mod, mod_factor, stream=None):
+    """eltwise-fma-mod.hpp:15; op3 may be None."""
+    s = _load().axhel_hip_fma_mod(_ptr(res), _ptr(op1), int(op2), _ptr(op3), n, mod, mod_factor,
+                                  _stream_ptr(stream))
+    if s == UNSUPPORTED_FACTOR:
+        return False
+    _check(s, "EltwiseFMAMod")
+    return True
+
+
+def EltwiseReduceMod(res, op, n, mod, in_mod_factor, out_mod_factor, stream=None):
+    """eltwise-reduce-mod.hpp:14."""
+    s = _load().axhel_hip_reduce_mod(_ptr(res), _ptr(op), n, mod, in_mod_factor, out_mod_factor,
+                                     _stream_ptr(stream))
+    if s == UNSUPPORTED_FACTOR:
+        return False
+    _check(s, "EltwiseReduceMod")
+    return True
+
+
+# --------------------------------------------------------------------- NTT
+class NTT:
+    """ntt.hpp:24-65: NTT(degree, modulus, root_of_unity) or
+    NTT(degree, modulus, root_powers=..., inv_root_powers=..., inv_degree_modulo=(op, quot)),
+    tables as uint64 arrays of 2*degree words ({operand, quotient} pairs)."""
+
+    def __init__(self, degree, modulus, root_of_unity=None, root_powers=None, inv_root_powers=None,
+                 inv_degree_modulo=None):
+        L = _load()
+        h = _p()
+        if root_of_unity is not None:
+            s = L.axhel_hip_ntt_plan_create(degree, modulus, root_of_unity, ctypes.byref(h))
+        else:
+            rp = np.ascontiguousarray(root_powers, dtype=np.uint64)
+            ip = np.ascontiguousarray(inv_root_powers, dtype=np.uint64)
+            s = L.axhel_hip_ntt_plan_create_external(degree, modulus, rp.ctypes.data, ip.ctypes.data,
+                                                     int(inv_degree_modulo[0]), int(inv_degree_modulo[1]),
+                                                     ctypes.byref(h))
+        _check(s, "NTT")
+        self._h = h
+        self.degree = int(degree)
+        self.modulus = int(modulus)
+
+    @property
+    def handle(self):
+        return self._h.value
+
+    def Degree(self):
+        return int(_load().axhel_hip_ntt_plan_degree(self._h))
+
+    def Modulus(self):
+        return int(_load().axhel_hip_ntt_plan_modulus(self._h))
+
+    def tables(self):
+        n = self.degree
+        fwd = np.zeros(2 * n, dtype=np.uint64)
+        inv = np.zeros(2 * n, dtype=np.uint64)
+        ni = np.zeros(2, dtype=np.uint64)
+        _check(_load().axhel_hip_ntt_plan_tables(self._h, fwd.ctypes.data, inv.ctypes.data, ni.ctypes.data),
+               "tables")
+        return fwd, inv, ni
+
+    def ComputeForward(self, result, operand, input_mod_factor=1, output_mod_factor=1, stream=None):
+        _check(_load().axhel_hip_ntt_forward(self._h, _ptr(result), _ptr(operand), input_mod_factor,
+                                             output_mod_factor, _stream_ptr(stream)), "ComputeForward")
+
+    def ComputeInverse(self, result, operand, input_mod_factor=1, output_mod_factor=1, stream=None):
+        _check(_load().axhel_hip_ntt_inverse(self._h, _ptr(result), _ptr(operand), input_mod_factor,
+                                             output_mod_factor, _stream_ptr(stream)), "ComputeInverse")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _load().axhel_hip_ntt_plan_destroy(self._h)
+            self._h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- batching
+class Batch:
+    """A reusable batch of limb-transforms captured into one hipGraph.
+
+    jobs: iterable of (ntt, result_tensor, operand_tensor, inverse: bool,
+    output_mod_factor) with device tensors of ntt.degree words each (views into
+    one big tensor are fine)."""
+
+    def __init__(self, jobs):
+        jobs = list(jobs)
+        arr = (NttJob * max(1, len(jobs)))()
+        self._keep = []
+        for i, (ntt, res, op, inverse, out_f) in enumerate(jobs):
+            arr[i].plan = ntt.handle
+            arr[i].result = res if isinstance(res, int) else _ptr(res)
+            arr[i].operand = op if isinstance(op, int) else _ptr(op)
+            arr[i].output_mod_factor = out_f
+            arr[i].inverse = 1 if inverse else 0
+            self._keep.append(ntt)
+        h = _p()
+        _check(_load().axhel_hip_ntt_batch_create(arr, len(jobs), ctypes.byref(h)), "batch_create")
+        self._h = h
+
+    def launch(self, stream=None):
+        _check(_load().axhel_hip_ntt_batch_launch(self._h, _stream_ptr(stream)), "batch_launch")
+
+    def launch_eager(self, stream=None):
+        _check(_load().axhel_hip_ntt_batch_launch_eager(self._h, _stream_ptr(stream)), "batch_launch_eager")
+
+    def kernel_count(self):
+        return int(_load().axhel_hip_ntt_batch_kernel_count(self._h))
+
+    def launch_kernel(self, index, stream=None):
+        _check(_load().axhel_hip_ntt_batch_launch_kernel(self._h, index, _stream_ptr(stream)), "launch_kernel")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _load().axhel_hip_ntt_batch_destroy(self._h)
+            self._h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ tuning
+def config_count(degree):
+    return int(_load().axhel_hip_ntt_config_count(degree))
+
+
+def config_get(degree, index):
+    c = NttConfig()
+    _check(_load().axhel_hip_ntt_config_get(degree, index, ctypes.byref(c)), "config_get")
+    return c.as_dict()
+
+
+def config_select(degree, index):
+    _check(_load().axhel_hip_ntt_config_select(degree, index), "config_select")
+
+
+def config_selected(degree):
+    return int(_load().axhel_hip_ntt_config_selected(degree))

@@ -1,0 +1,101 @@
+// Exact 64-bit modular primitives for gfx950 device code.
+//
+// These are the device forms of the scalar routines the reference's Native
+// backend uses (axhel/include/axhel/number-theory/uint-arith.hpp:19-60 and
+// modular-reduction.hpp:22-67).  Each returns the same 64-bit value as the
+// reference for every input, including out-of-range ones: products are exact
+// hi64/lo64 parts, never approximations, because lazy NTT outputs depend on
+// the exact Shoup quotient.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace axhel_dev {
+
+// x >= b ? x - b : x  (modular-reduction.hpp:22-25).
+__device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t b) {
+    const uint64_t d = x - b;
+    return x >= b ? d : x;
+}
+
+// ReduceInputNative<F> (modular-reduction.hpp:48-67): cascaded conditional
+// subtractions 4q, 2q, q for F = 8; 2q, q for F = 4; q for F = 2.
+template <int F>
+__device__ __forceinline__ uint64_t reduce_input(uint64_t x, uint64_t q) {
+    if constexpr (F == 8) x = csub(x, q << 2);
+    if constexpr (F >= 4) x = csub(x, q << 1);
+    if constexpr (F >= 2) x = csub(x, q);
+    return x;
+}
+
+// Shoup lazy product y*w - hi64(y*w')*q mod 2^64, in [0, 2q) when w < q and
+// w' = floor(w * 2^64 / q) (uint-arith.hpp:48-52).
+__device__ __forceinline__ uint64_t shoup(uint64_t y, uint64_t w, uint64_t wq, uint64_t q) {
+    const uint64_t qhat = __umul64hi(y, wq);
+    return y * w - qhat * q;
+}
+
+// Harvey forward (Cooley-Tukey) butterfly, ntt-internal.hpp:20-31.
+__device__ __forceinline__ void fwd_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wq, uint64_t q,
+                                         uint64_t q2) {
+    const uint64_t xh = csub(x, q2);
+    const uint64_t t = shoup(y, w, wq, q);
+    x = xh + t;
+    y = xh + q2 - t;
+}
+
+// Harvey inverse (Gentleman-Sande) butterfly, ntt-internal.hpp:37-49.
+__device__ __forceinline__ void inv_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wq, uint64_t q,
+                                         uint64_t q2) {
+    const uint64_t u = x, v = y;
+    x = csub(u + v, q2);
+    y = shoup(u + q2 - v, w, wq, q);
+}
+
+// Final inverse stage butterfly with n^-1 and the scaled root folded in,
+// ntt-native.cpp:126-141.
+__device__ __forceinline__ void inv_final_bfly(uint64_t &x, uint64_t &y, uint64_t ni, uint64_t niq, uint64_t s,
+                                               uint64_t sq, uint64_t q, uint64_t q2) {
+    const uint64_t xh = csub(x, q2);
+    const uint64_t sum = xh + y;
+    const uint64_t dif = xh + q2 - y;
+    x = shoup(sum, ni, niq, q);
+    y = shoup(dif, s, sq, q);
+}
+
+// Barrett constants of eltwise-mul-mod.cpp:22-30 (n = BitWidth(q),
+// s = n - 2, mu = floor(2^(n-2) * 2^64 / q)), computed once on the host.
+struct Barrett {
+    uint64_t q;
+    uint64_t mu;
+    uint32_t s;
+    uint32_t s_comp;  // (64 - s) & 63: the reference's hi << (64 - s) taken mod 64
+};
+
+// 128-bit (hi:lo) >> s as at eltwise-mul-mod.cpp:37.  For s == 0 the
+// reference's `hi << 64` is a register shift, which both AArch64 and x86-64
+// reduce modulo 64 (shift by 0); s_comp carries that reduction.
+__device__ __forceinline__ uint64_t shr128(uint64_t hi, uint64_t lo, const Barrett &b) {
+    return (lo >> b.s) | (hi << b.s_comp);
+}
+
+// Barrett product of x, y < q, eltwise-mul-mod.cpp:34-61: z in [0, 4q) then
+// reduced to [0, q).
+__device__ __forceinline__ uint64_t barrett_mul(uint64_t x, uint64_t y, const Barrett &b) {
+    const uint64_t lo = x * y;
+    const uint64_t hi = __umul64hi(x, y);
+    const uint64_t c1 = shr128(hi, lo, b);
+    const uint64_t qhat = __umul64hi(c1, b.mu);
+    const uint64_t z = lo - qhat * b.q;
+    return reduce_input<4>(z, b.q);
+}
+
+// Barrett reduction of an arbitrary 64-bit word, eltwise-reduce-mod.cpp:19-29.
+__device__ __forceinline__ uint64_t barrett_reduce64(uint64_t x, const Barrett &b) {
+    const uint64_t qhat = __umul64hi(x >> b.s, b.mu);
+    const uint64_t z = x - qhat * b.q;
+    return reduce_input<4>(z, b.q);
+}
+
+}  // namespace axhel_dev

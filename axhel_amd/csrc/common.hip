@@ -1,0 +1,149 @@
+// Host-side helpers: error text, device queries, host-pointer staging.
+#include "common.hpp"
+
+#include <cstdio>
+
+namespace axhel_host {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const char *where, hipError_t err) {
+    g_last_error = std::string(where) + ": " + hipGetErrorString(err);
+}
+
+void set_last_error_text(const std::string &text) { g_last_error = text; }
+
+bool is_device_pointer(const void *p) {
+    if (p == nullptr) return false;
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof(attr));
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // unregistered host memory: clear the error
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+int cu_count() {
+    static std::mutex mu;
+    static std::vector<int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+    if (cache[dev] == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cache[dev] = v;
+    }
+    return cache[dev];
+}
+
+Staging::~Staging() {
+    for (auto &e : entries_) (void)hipFree(e.dev);
+}
+
+hipError_t Staging::stage(const void *p, uint64_t n, bool copy_in, void **dev, Entry **entry) {
+    for (auto &e : entries_) {
+        if (e.host == p && e.n == n) {
+            *dev = e.dev;
+            *entry = &e;
+            return hipSuccess;
+        }
+    }
+    void *d = nullptr;
+    hipError_t err = hipMalloc(&d, n ? n * sizeof(uint64_t) : sizeof(uint64_t));
+    if (err != hipSuccess) return err;
+    if (copy_in && n) {
+        err = hipMemcpyAsync(d, p, n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_);
+        if (err != hipSuccess) {
+            (void)hipFree(d);
+            return err;
+        }
+    }
+    entries_.push_back(Entry{p, d, n, false});
+    *dev = d;
+    *entry = &entries_.back();
+    return hipSuccess;
+}
+
+hipError_t Staging::input(const uint64_t *p, uint64_t n, const uint64_t **dev) {
+    if (p == nullptr || is_device_pointer(p)) {
+        *dev = p;
+        return hipSuccess;
+    }
+    void *d = nullptr;
+    Entry *e = nullptr;
+    hipError_t err = stage(p, n, true, &d, &e);
+    *dev = static_cast<const uint64_t *>(d);
+    return err;
+}
+
+hipError_t Staging::output(uint64_t *p, uint64_t n, uint64_t **dev) {
+    if (p == nullptr || is_device_pointer(p)) {
+        *dev = p;
+        return hipSuccess;
+    }
+    void *d = nullptr;
+    Entry *e = nullptr;
+    // The result buffer is fully overwritten by every kernel, so a fresh
+    // staging buffer needs no copy-in unless it aliases an operand (handled
+    // by stage() returning the operand's buffer).
+    hipError_t err = stage(p, n, false, &d, &e);
+    if (err == hipSuccess) e->is_out = true;
+    *dev = static_cast<uint64_t *>(d);
+    return err;
+}
+
+hipError_t Staging::finish() {
+    if (entries_.empty()) return hipSuccess;
+    for (auto &e : entries_) {
+        if (!e.is_out || e.n == 0) continue;
+        hipError_t err = hipMemcpyAsync(const_cast<void *>(e.host), e.dev, e.n * sizeof(uint64_t),
+                                        hipMemcpyDeviceToHost, stream_);
+        if (err != hipSuccess) return err;
+    }
+    return hipStreamSynchronize(stream_);
+}
+
+}  // namespace axhel_host
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char *axhel_hip_status_string(axhel_hip_status s) {
+    switch (s) {
+        case AXHEL_HIP_OK: return "ok";
+        case AXHEL_HIP_UNSUPPORTED_FACTOR: return "unsupported mod factor (no-op)";
+        case AXHEL_HIP_INVALID_ARGUMENT: return "invalid argument";
+        case AXHEL_HIP_NO_MEMORY: return "out of device memory";
+        case AXHEL_HIP_HIP_ERROR: return "HIP runtime error";
+        case AXHEL_HIP_NO_DEVICE: return "no GPU device";
+        case AXHEL_HIP_REJECTED: return "request rejected";
+        case AXHEL_HIP_TIMEOUT: return "timed out";
+        case AXHEL_HIP_CANCELLED: return "cancelled";
+    }
+    return "unknown status";
+}
+
+const char *axhel_hip_last_error(void) { return axhel_host::g_last_error.c_str(); }
+
+int axhel_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int axhel_hip_is_device_pointer(const void *p) { return axhel_host::is_device_pointer(p) ? 1 : 0; }
+
+axhel_hip_status axhel_hip_stream_synchronize(void *stream) {
+    AXHEL_HIP_TRY(hipStreamSynchronize(axhel_host::as_stream(stream)));
+    return AXHEL_HIP_OK;
+}
+
+}  // extern "C"

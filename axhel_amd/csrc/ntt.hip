@@ -1,0 +1,397 @@
+// NTT plans, twiddle tables, kernel configurations and the plain per-call
+// C-ABI entry points (ComputeForward / ComputeInverse drop-ins).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "ntt_host.hpp"
+
+using axhel_dev::NttJobDev;
+using axhel_dev::PassArgs;
+
+namespace axhel_host {
+namespace {
+
+typedef unsigned __int128 u128;
+
+uint64_t mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)(((u128)a * b) % q); }
+
+uint64_t powmod(uint64_t b, uint64_t e, uint64_t q) {
+    uint64_t r = 1 % q;
+    b %= q;
+    while (e) {
+        if (e & 1) r = mulmod(r, b, q);
+        e >>= 1;
+        if (e) b = mulmod(b, b, q);
+    }
+    return r;
+}
+
+uint64_t shoup_quotient(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 64) / q); }
+
+// Twiddle tables of ntt-tables.cpp:81-140: fwd[i] = psi^brv(i);
+// inv[0] = {0, 0}, inv[i] = psi^-(brv(i-1)+1); n^-1 = n^(q-2).  Bit reversal
+// is computed directly rather than by the reference's incremental counter;
+// the index sequence is the same.
+void gen_tables(uint32_t logn, uint64_t q, uint64_t psi, std::vector<uint64_t> &fwd, std::vector<uint64_t> &inv,
+                uint64_t &ninv) {
+    const uint64_t n = 1ull << logn;
+    auto brv = [logn](uint64_t i) -> uint64_t {
+        uint64_t r = 0;
+        for (uint32_t b = 0; b < logn; ++b) r |= ((i >> b) & 1ull) << (logn - 1 - b);
+        return r;
+    };
+    std::vector<uint64_t> pw(n);
+    pw[0] = 1 % q;
+    for (uint64_t k = 1; k < n; ++k) pw[k] = mulmod(pw[k - 1], psi, q);
+    fwd.assign(2 * n, 0);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t w = pw[brv(i)];
+        fwd[2 * i] = w;
+        fwd[2 * i + 1] = shoup_quotient(w, q);
+    }
+    const uint64_t ipsi = powmod(psi, q - 2, q);
+    pw[0] = 1 % q;
+    for (uint64_t k = 1; k < n; ++k) pw[k] = mulmod(pw[k - 1], ipsi, q);
+    inv.assign(2 * n, 0);
+    for (uint64_t i = 1; i < n; ++i) {
+        const uint64_t w = pw[brv(i - 1) + 1];
+        inv[2 * i] = w;
+        inv[2 * i + 1] = shoup_quotient(w, q);
+    }
+    ninv = powmod(n % q, q - 2, q);
+}
+
+// ------------------------------------------------------------ configurations
+
+template <int LG, int LE, int GPW, bool COL, bool MULTI, bool FWD>
+PassKernel pk() {
+    PassKernel k;
+    k.fn = reinterpret_cast<const void *>(&axhel_dev::ntt_pass_kernel<LG, LE, GPW, FWD, COL, MULTI>);
+    k.log_g = LG;
+    k.log_e = LE;
+    k.gpw = GPW;
+    k.col = COL;
+    k.multi = MULTI;
+    k.threads = GPW << (LG - LE);
+    k.lds_bytes = (size_t)axhel_dev::Lds<LG, LE, GPW, COL>::WORDS * sizeof(uint64_t);
+    return k;
+}
+
+template <int LG, int LE, int GPW>
+NttConfig single() {
+    NttConfig c;
+    c.pub.log_elems = LE;
+    c.pub.row_groups = GPW;
+    c.pub.col_groups = 0;
+    c.pub.log_col = 0;
+    c.npass = 1;
+    c.fwd[0] = pk<LG, LE, GPW, false, (GPW > 1), true>();
+    c.inv[0] = pk<LG, LE, GPW, false, (GPW > 1), false>();
+    return c;
+}
+
+// Two passes: COL over the top LC bits, ROW over the low LR bits.
+template <int LC, int LEC, int GC, int LR, int LER, int GR>
+NttConfig two() {
+    NttConfig c;
+    c.pub.log_elems = LER;
+    c.pub.row_groups = GR;
+    c.pub.col_groups = GC;
+    c.pub.log_col = LC;
+    c.npass = 2;
+    c.fwd[0] = pk<LC, LEC, GC, true, false, true>();
+    c.fwd[1] = pk<LR, LER, GR, false, false, true>();
+    c.inv[0] = pk<LR, LER, GR, false, false, false>();
+    c.inv[1] = pk<LC, LEC, GC, true, false, false>();
+    return c;
+}
+
+std::vector<std::vector<NttConfig>> build_registry() {
+    std::vector<std::vector<NttConfig>> r(kMaxLogN + 1);
+    r[1] = {single<1, 1, 256>()};
+    r[2] = {single<2, 2, 256>()};
+    r[3] = {single<3, 3, 256>()};
+    r[4] = {single<4, 4, 256>()};
+    r[5] = {single<5, 4, 128>(), single<5, 3, 64>()};
+    r[6] = {single<6, 4, 64>(), single<6, 3, 32>()};
+    r[7] = {single<7, 4, 32>(), single<7, 3, 16>()};
+    r[8] = {single<8, 4, 16>(), single<8, 3, 8>()};
+    r[9] = {single<9, 4, 8>(), single<9, 3, 4>()};
+    r[10] = {single<10, 4, 4>(), single<10, 3, 2>()};
+    r[11] = {single<11, 4, 2>(), single<11, 3, 1>()};
+    r[12] = {single<12, 4, 1>(), single<12, 3, 1>()};
+    r[13] = {single<13, 4, 1>(), single<13, 3, 1>()};
+    r[14] = {two<7, 4, 32, 7, 4, 32>(), two<7, 4, 16, 7, 4, 16>()};
+    r[15] = {two<8, 4, 16, 7, 4, 32>(), two<7, 4, 32, 8, 4, 16>()};
+    r[16] = {two<8, 4, 16, 8, 4, 16>(), two<8, 4, 8, 8, 4, 8>(), two<8, 3, 16, 8, 3, 8>()};
+    r[17] = {two<9, 3, 8, 8, 4, 16>(), two<8, 4, 16, 9, 3, 8>()};
+    return r;
+}
+
+std::once_flag g_reg_once;
+std::vector<std::vector<NttConfig>> *g_reg = nullptr;
+std::atomic<int> g_selected[kMaxLogN + 1];
+
+const std::vector<std::vector<NttConfig>> &registry() {
+    std::call_once(g_reg_once, [] {
+        g_reg = new std::vector<std::vector<NttConfig>>(build_registry());
+        for (auto &s : g_selected) s.store(0);
+    });
+    return *g_reg;
+}
+
+// Kernels using more than the default 64 KiB of dynamic LDS need the limit
+// raised once per kernel and device.
+std::mutex g_attr_mu;
+std::vector<std::pair<const void *, int>> g_attr_done;
+
+hipError_t ensure_lds(const PassKernel &k) {
+    if (k.lds_bytes <= 65536) return hipSuccess;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_attr_mu);
+    for (auto &d : g_attr_done)
+        if (d.first == k.fn && d.second == dev) return hipSuccess;
+    e = hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.lds_bytes);
+    if (e == hipSuccess) g_attr_done.emplace_back(k.fn, dev);
+    return e;
+}
+
+}  // namespace
+
+const std::vector<NttConfig> &configs_for(uint32_t logn) {
+    static const std::vector<NttConfig> empty;
+    if (logn == 0 || logn > kMaxLogN) return empty;
+    return registry()[logn];
+}
+
+const NttConfig &config_for(uint32_t logn) {
+    const auto &v = configs_for(logn);
+    int i = g_selected[logn].load();
+    if (i < 0 || i >= (int)v.size()) i = 0;
+    return v[i];
+}
+
+hipError_t prepare_config(const NttConfig &cfg) {
+    for (int p = 0; p < cfg.npass; ++p) {
+        hipError_t e = ensure_lds(cfg.fwd[p]);
+        if (e == hipSuccess) e = ensure_lds(cfg.inv[p]);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+NttJobDev make_job(const axhel_hip_plan *p, const uint64_t *src, uint64_t *dst, bool inverse) {
+    NttJobDev j;
+    j.src = src;
+    j.dst = dst;
+    j.tw = inverse ? p->d_inv : p->d_fwd;
+    j.q = p->q;
+    j.ninv = p->ninv;
+    j.ninv_q = p->ninv_q;
+    j.sroot = p->sroot;
+    j.sroot_q = p->sroot_q;
+    return j;
+}
+
+hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool normalize, const NttJobDev *jobs,
+                      const NttJobDev &inline_job, uint32_t count, hipStream_t stream, int only_pass) {
+    if (count == 0) return hipSuccess;
+    const uint64_t n = 1ull << logn;
+    for (int p = 0; p < cfg.npass; ++p) {
+        if (only_pass >= 0 && p != only_pass) continue;
+        const PassKernel &k = inverse ? cfg.inv[p] : cfg.fwd[p];
+        hipError_t e = ensure_lds(k);
+        if (e != hipSuccess) return e;
+        PassArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.jobs = jobs;
+        a.inline_job = inline_job;
+        a.logn = logn;
+        a.num_limbs = count;
+        a.groups_per_limb = (uint32_t)(n >> k.log_g);
+        a.first_pass = p == 0;
+        a.normalize = (p == cfg.npass - 1) && normalize;
+        a.final_inv = inverse && (p == cfg.npass - 1);
+        uint64_t blocks;
+        if (k.multi) blocks = (count + k.gpw - 1) / k.gpw;
+        else blocks = (uint64_t)count * a.groups_per_limb / k.gpw;
+        void *args[] = {&a};
+        e = hipLaunchKernel(k.fn, dim3((unsigned)blocks), dim3(k.threads), args, k.lds_bytes, stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace axhel_host
+
+// ------------------------------------------------------------------ C ABI
+
+using namespace axhel_host;
+
+namespace {
+
+axhel_hip_status finish_plan(axhel_hip_plan *p, axhel_hip_plan **out) {
+    // ntt-native.cpp:109-113: the final inverse stage's scaled root, computed
+    // once per plan instead of once per call (same value).
+    const uint64_t wl = p->h_inv[2 * (p->n - 1)];
+    uint64_t s = wl * p->ninv - (uint64_t)(((u128)wl * p->ninv_q) >> 64) * p->q;
+    if (s >= p->q) s -= p->q;
+    p->sroot = s;
+    p->sroot_q = shoup_quotient(s, p->q);
+    hipError_t e = hipGetDevice(&p->device);
+    if (e == hipSuccess) e = hipMalloc(&p->d_fwd, p->n * sizeof(ulonglong2));
+    if (e == hipSuccess) e = hipMalloc(&p->d_inv, p->n * sizeof(ulonglong2));
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_fwd, p->h_fwd.data(), p->n * sizeof(ulonglong2), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_inv, p->h_inv.data(), p->n * sizeof(ulonglong2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_last_error("plan upload", e);
+        (void)hipFree(p->d_fwd);
+        (void)hipFree(p->d_inv);
+        delete p;
+        return e == hipErrorOutOfMemory ? AXHEL_HIP_NO_MEMORY : AXHEL_HIP_HIP_ERROR;
+    }
+    *out = p;
+    return AXHEL_HIP_OK;
+}
+
+bool valid_degree(uint64_t degree, uint32_t *logn) {
+    if (degree < 2 || (degree & (degree - 1)) != 0) return false;
+    uint32_t l = 0;
+    while ((1ull << l) < degree) ++l;
+    if (l > kMaxLogN) return false;
+    *logn = l;
+    return true;
+}
+
+axhel_hip_status run_one(const axhel_hip_plan *plan, uint64_t *result, const uint64_t *operand, bool inverse,
+                         bool normalize, void *stream) {
+    if (!plan || !result || !operand) return AXHEL_HIP_INVALID_ARGUMENT;
+    const hipStream_t st = as_stream(stream);
+    Staging stg(st);
+    const uint64_t *dop = nullptr;
+    uint64_t *dres = nullptr;
+    AXHEL_HIP_TRY(stg.input(operand, plan->n, &dop));
+    AXHEL_HIP_TRY(stg.output(result, plan->n, &dres));
+    const NttJobDev job = make_job(plan, dop, dres, inverse);
+    AXHEL_HIP_TRY(launch_ntt(config_for(plan->logn), plan->logn, inverse, normalize, nullptr, job, 1, st));
+    AXHEL_HIP_TRY(stg.finish());
+    return AXHEL_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+axhel_hip_status axhel_hip_ntt_plan_create(uint64_t degree, uint64_t modulus, uint64_t root_of_unity,
+                                           axhel_hip_plan **plan) {
+    if (!plan) return AXHEL_HIP_INVALID_ARGUMENT;
+    *plan = nullptr;
+    uint32_t logn = 0;
+    if (!valid_degree(degree, &logn) || modulus < 2 || modulus >= (1ull << 62)) return AXHEL_HIP_INVALID_ARGUMENT;
+    if (axhel_hip_device_count() == 0) return AXHEL_HIP_NO_DEVICE;
+    axhel_hip_plan *p = new (std::nothrow) axhel_hip_plan;
+    if (!p) return AXHEL_HIP_NO_MEMORY;
+    p->n = degree;
+    p->logn = logn;
+    p->q = modulus;
+    gen_tables(logn, modulus, root_of_unity, p->h_fwd, p->h_inv, p->ninv);
+    p->ninv_q = shoup_quotient(p->ninv, modulus);
+    return finish_plan(p, plan);
+}
+
+axhel_hip_status axhel_hip_ntt_plan_create_external(uint64_t degree, uint64_t modulus, const uint64_t *root_powers,
+                                                    const uint64_t *inv_root_powers, uint64_t inv_degree_operand,
+                                                    uint64_t inv_degree_quotient, axhel_hip_plan **plan) {
+    if (!plan) return AXHEL_HIP_INVALID_ARGUMENT;
+    *plan = nullptr;
+    uint32_t logn = 0;
+    if (!valid_degree(degree, &logn) || modulus < 2 || modulus >= (1ull << 62) || !root_powers || !inv_root_powers)
+        return AXHEL_HIP_INVALID_ARGUMENT;
+    if (axhel_hip_device_count() == 0) return AXHEL_HIP_NO_DEVICE;
+    axhel_hip_plan *p = new (std::nothrow) axhel_hip_plan;
+    if (!p) return AXHEL_HIP_NO_MEMORY;
+    p->n = degree;
+    p->logn = logn;
+    p->q = modulus;
+    p->h_fwd.assign(root_powers, root_powers + 2 * degree);
+    p->h_inv.assign(inv_root_powers, inv_root_powers + 2 * degree);
+    p->ninv = inv_degree_operand;
+    p->ninv_q = inv_degree_quotient;
+    return finish_plan(p, plan);
+}
+
+axhel_hip_status axhel_hip_ntt_plan_destroy(axhel_hip_plan *plan) {
+    if (!plan) return AXHEL_HIP_OK;
+    (void)hipFree(plan->d_fwd);
+    (void)hipFree(plan->d_inv);
+    delete plan;
+    return AXHEL_HIP_OK;
+}
+
+uint64_t axhel_hip_ntt_plan_degree(const axhel_hip_plan *plan) { return plan ? plan->n : 0; }
+uint64_t axhel_hip_ntt_plan_modulus(const axhel_hip_plan *plan) { return plan ? plan->q : 0; }
+
+axhel_hip_status axhel_hip_ntt_plan_tables(const axhel_hip_plan *plan, uint64_t *root_powers,
+                                           uint64_t *inv_root_powers, uint64_t *inv_degree) {
+    if (!plan) return AXHEL_HIP_INVALID_ARGUMENT;
+    if (root_powers) std::memcpy(root_powers, plan->h_fwd.data(), plan->h_fwd.size() * sizeof(uint64_t));
+    if (inv_root_powers) std::memcpy(inv_root_powers, plan->h_inv.data(), plan->h_inv.size() * sizeof(uint64_t));
+    if (inv_degree) {
+        inv_degree[0] = plan->ninv;
+        inv_degree[1] = plan->ninv_q;
+    }
+    return AXHEL_HIP_OK;
+}
+
+axhel_hip_status axhel_hip_ntt_forward(const axhel_hip_plan *plan, uint64_t *result, const uint64_t *operand,
+                                       uint64_t input_mod_factor, uint64_t output_mod_factor, void *stream) {
+    (void)input_mod_factor;  // ignored, as in ntt.cpp:173-186
+    return run_one(plan, result, operand, false, output_mod_factor != 4, stream);
+}
+
+axhel_hip_status axhel_hip_ntt_inverse(const axhel_hip_plan *plan, uint64_t *result, const uint64_t *operand,
+                                       uint64_t input_mod_factor, uint64_t output_mod_factor, void *stream) {
+    (void)input_mod_factor;  // ignored, as in ntt.cpp:192-204
+    return run_one(plan, result, operand, true, output_mod_factor != 2, stream);
+}
+
+int axhel_hip_ntt_config_count(uint64_t degree) {
+    uint32_t logn = 0;
+    if (!valid_degree(degree, &logn)) return 0;
+    return (int)configs_for(logn).size();
+}
+
+axhel_hip_status axhel_hip_ntt_config_get(uint64_t degree, int index, axhel_hip_ntt_config *config) {
+    uint32_t logn = 0;
+    if (!config || !valid_degree(degree, &logn)) return AXHEL_HIP_INVALID_ARGUMENT;
+    const auto &v = configs_for(logn);
+    if (index < 0 || index >= (int)v.size()) return AXHEL_HIP_INVALID_ARGUMENT;
+    *config = v[index].pub;
+    return AXHEL_HIP_OK;
+}
+
+axhel_hip_status axhel_hip_ntt_config_select(uint64_t degree, int index) {
+    uint32_t logn = 0;
+    if (!valid_degree(degree, &logn)) return AXHEL_HIP_INVALID_ARGUMENT;
+    const auto &v = configs_for(logn);
+    if (index < -1 || index >= (int)v.size()) return AXHEL_HIP_INVALID_ARGUMENT;
+    g_selected[logn].store(index < 0 ? 0 : index);
+    return AXHEL_HIP_OK;
+}
+
+int axhel_hip_ntt_config_selected(uint64_t degree) {
+    uint32_t logn = 0;
+    if (!valid_degree(degree, &logn)) return -1;
+    (void)registry();
+    return g_selected[logn].load();
+}
+
+}  // extern "C"

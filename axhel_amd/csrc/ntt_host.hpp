@@ -1,0 +1,69 @@
+// Host-side NTT internals shared by the plain, batched and serving paths.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "common.hpp"
+#include "ntt_kernels.hpp"
+
+// Plan: one (degree, modulus, root) with resident device tables.
+struct axhel_hip_plan {
+    uint64_t n = 0;
+    uint32_t logn = 0;
+    uint64_t q = 0;
+    int device = 0;
+    std::vector<uint64_t> h_fwd, h_inv;  // AoS {operand, quotient}, 2n words each
+    uint64_t ninv = 0, ninv_q = 0;
+    uint64_t sroot = 0, sroot_q = 0;  // CS(Shoup(inv[n-1], n^-1), q) and its quotient
+    ulonglong2 *d_fwd = nullptr;
+    ulonglong2 *d_inv = nullptr;
+};
+
+namespace axhel_host {
+
+constexpr uint32_t kMaxLogN = 17;
+
+// One compiled pass kernel.
+struct PassKernel {
+    const void *fn = nullptr;
+    int log_g = 0;
+    int log_e = 0;
+    int gpw = 0;
+    bool col = false;
+    bool multi = false;
+    int threads = 0;
+    size_t lds_bytes = 0;
+};
+
+// One transform configuration for a degree: the passes in execution order.
+struct NttConfig {
+    axhel_hip_ntt_config pub{};
+    int npass = 0;
+    PassKernel fwd[2];
+    PassKernel inv[2];
+};
+
+// Compiled configurations for log2(degree) (empty for unsupported degrees).
+const std::vector<NttConfig> &configs_for(uint32_t logn);
+// Configuration currently selected for log2(degree).
+const NttConfig &config_for(uint32_t logn);
+
+// Raises per-kernel limits (dynamic LDS above 64 KiB) for every pass of a
+// configuration; done before graph capture so capture records launches only.
+hipError_t prepare_config(const NttConfig &cfg);
+
+// Device job descriptor for plan + pointers.
+axhel_dev::NttJobDev make_job(const axhel_hip_plan *p, const uint64_t *src, uint64_t *dst, bool inverse);
+
+// Launches all passes of one transform family: `count` limbs of the same
+// degree and direction whose descriptors are `jobs` (device table) or, when
+// jobs == nullptr and count == 1, `inline_job`.  only_pass >= 0 launches
+// that pass alone (per-kernel timing).
+hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool normalize,
+                      const axhel_dev::NttJobDev *jobs, const axhel_dev::NttJobDev &inline_job, uint32_t count,
+                      hipStream_t stream, int only_pass = -1);
+
+}  // namespace axhel_host

@@ -1,0 +1,347 @@
+// Negacyclic Harvey NTT pass kernels for gfx950.
+//
+// A length-N transform (N = 2^logN, uint64 words) applies radix-2 Harvey
+// butterflies over the bit positions of the coefficient index: the forward
+// (Cooley-Tukey) transform from bit logN-1 down to bit 0, the inverse
+// (Gentleman-Sande) from bit 0 up to bit logN-1 (ntt-native.cpp:31-54 and
+// :72-141).  A butterfly on bit B pairs P and P + 2^B (bit B of P clear) and
+// uses twiddle
+//     forward:  root_powers[(N >> (B+1)) + (P >> (B+1))]        (= [m + i])
+//     inverse:  inv_root_powers[N + 1 - (N >> B) + (P >> (B+1))] (= [N-2m+1+i])
+// with m = N >> (B+1) blocks, i = P >> (B+1) the block index; the inverse's
+// last bit (B = logN-1) is the scaled final stage of ntt-native.cpp:101-141.
+//
+// Every butterfly is the reference's radix-2 formula on the reference's pair
+// with the reference's twiddle, and butterflies on one bit run only after all
+// butterflies on the previous bit that feed them, so outputs are bit-identical
+// to the scalar loop, lazy bits included.  Only the order of independent
+// butterflies changes.
+//
+// Mapping to the hardware.  A pass handles a contiguous range of LOG_G bit
+// positions.  Fixing every other index bit leaves a "group" of G = 2^LOG_G
+// words that the pass transforms independently:
+//   ROW pass: bits [0, LOG_G) -> the group is G contiguous words;
+//   COL pass: bits [logN-LOG_G, logN) -> the group is G words at stride
+//             2^(logN-LOG_G) (a column of the N/G x G row-major matrix).
+// N <= 2^13 runs as one ROW pass (the whole limb sits in LDS); larger N
+// runs a COL pass and a ROW pass (forward: COL then ROW; inverse: ROW then
+// COL), two HBM round trips in all.
+//
+// Inside a group, T = G/E threads each hold E = 2^LOG_E words in registers.
+// The pass's bits are cut into chunks of LOG_E bits (the last may be
+// shorter); a chunk's butterflies all run in registers, and between chunks the
+// group re-distributes through LDS so the next chunk's bits are register-
+// local.  A workgroup holds GPW groups: COL groups are adjacent columns, so a
+// wave's lanes walk adjacent addresses; ROW groups are adjacent rows.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "arith.hpp"
+
+namespace axhel_dev {
+
+// Device descriptor of one limb-transform.
+struct NttJobDev {
+    const uint64_t *src;    // operand (read by the first pass)
+    uint64_t *dst;          // result (written by every pass; may equal src)
+    const ulonglong2 *tw;   // forward or inverse table, {operand, quotient} AoS
+    uint64_t q;
+    uint64_t ninv, ninv_q;  // n^-1 and its Shoup quotient (inverse final stage)
+    uint64_t sroot, sroot_q;  // scaled final root CS(Shoup(W_last, n^-1), q) and quotient
+};
+
+struct PassArgs {
+    const NttJobDev *jobs;  // device job table; null -> inline_job (single limb)
+    NttJobDev inline_job;
+    uint32_t logn;
+    uint32_t num_limbs;
+    uint32_t groups_per_limb;  // N >> LOG_G (rows of a ROW pass, columns of a COL pass)
+    uint32_t first_pass;       // read the job's src (else dst)
+    uint32_t normalize;        // last pass: fold the [0,4q)->[0,q) / [0,2q)->[0,q) reduction in
+    uint32_t final_inv;        // inverse pass holding bit logN-1 (scaled final stage)
+};
+
+__device__ __forceinline__ NttJobDev load_job(const PassArgs &a, uint32_t limb) {
+    return a.jobs ? a.jobs[limb] : a.inline_job;
+}
+
+__device__ __forceinline__ const uint64_t *job_src(const PassArgs &a, uint32_t limb) {
+    if (a.jobs) return a.first_pass ? a.jobs[limb].src : a.jobs[limb].dst;
+    return a.first_pass ? a.inline_job.src : a.inline_job.dst;
+}
+
+__device__ __forceinline__ uint64_t *job_dst(const PassArgs &a, uint32_t limb) {
+    return a.jobs ? a.jobs[limb].dst : a.inline_job.dst;
+}
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// Geometry of chunk C of a pass with LOG_G bits and LOG_E bits per chunk.
+template <int LOG_G, int LOG_E, bool FWD>
+struct Geom {
+    static constexpr int NC = (LOG_G + LOG_E - 1) / LOG_E;
+    static constexpr int LAST_B = LOG_G - (NC - 1) * LOG_E;
+    static constexpr int B(int c) { return c < NC - 1 ? LOG_E : LAST_B; }
+    // Forward chunks walk the local bits from the top, inverse from the bottom.
+    static constexpr int LO(int c) { return FWD ? LOG_G - (c * LOG_E + B(c)) : c * LOG_E; }
+    // Inserts the B-bit gap at position LO into an (LOG_G - B)-bit value.
+    static constexpr uint32_t spread(uint32_t o, int lo, int b) {
+        return (o & ((1u << lo) - 1u)) | ((o >> lo) << (lo + b));
+    }
+    // Local index of register r (compile-time part) in chunk c.
+    static constexpr uint32_t kr(int c, int r) {
+        const int b = B(c), lo = LO(c);
+        const uint32_t rlow = (uint32_t)r & ((1u << b) - 1u);
+        const uint32_t rhi = (uint32_t)r >> b;
+        return spread(rhi, lo, b) | (rlow << lo);
+    }
+    // Local index contribution of the in-group thread id in chunk c.
+    __device__ static uint32_t kt(int c, uint32_t tid) {
+        const int b = B(c), lo = LO(c);
+        return spread(tid << (LOG_E - b), lo, b);
+    }
+    // Smallest register sharing r's twiddle on local bit p of chunk c (the
+    // twiddle depends only on index bits above p).
+    static constexpr int rep(int c, int r, int p) {
+        const uint32_t key = kr(c, r) >> (p + 1);
+        const int bit = 1 << (p - LO(c));
+        for (int s = 0; s < (1 << LOG_E); ++s) {
+            if (s & bit) continue;
+            if ((kr(c, s) >> (p + 1)) == key) return s;
+        }
+        return r;
+    }
+};
+
+template <int LOG_G, int LOG_E, int GPW, bool COL>
+struct Lds {
+    static constexpr int G = 1 << LOG_G;
+    // ROW groups pad one word per E words so thread-contiguous chunks hit
+    // distinct banks; COL tiles interleave groups so lanes (adjacent groups)
+    // hit adjacent words.
+    static constexpr int ROW_STRIDE = G + (G >> LOG_E);
+    static constexpr int WORDS = COL ? GPW * G : GPW * ROW_STRIDE;
+    __device__ static uint32_t idx(uint32_t g, uint32_t k) {
+        if constexpr (COL) return k * GPW + g;
+        else return g * ROW_STRIDE + k + (k >> LOG_E);
+    }
+};
+
+template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, bool MULTI>
+__global__ void __launch_bounds__(GPW << (LOG_G - LOG_E))
+ntt_pass_kernel(PassArgs a) {
+    using Gm = Geom<LOG_G, LOG_E, FWD>;
+    using L = Lds<LOG_G, LOG_E, GPW, COL>;
+    constexpr int G = 1 << LOG_G;
+    constexpr int E = 1 << LOG_E;
+    constexpr int T = G / E;
+    constexpr int NT = GPW * T;
+    constexpr int NC = Gm::NC;
+    extern __shared__ uint64_t lds[];
+
+    // XCD-aware tile order: consecutive workgroup ids are spread round-robin
+    // over the 8 XCDs, so give each XCD a contiguous slice of the tile space
+    // (limbs sharing a twiddle table then share that XCD's L2).
+    uint32_t tile = blockIdx.x;
+    {
+        const uint32_t nb = gridDim.x;
+        if ((nb & 7u) == 0) tile = (tile & 7u) * (nb >> 3) + (tile >> 3);
+    }
+
+    const uint32_t t = threadIdx.x;
+    uint32_t g, tid;
+    if constexpr (COL) {
+        g = t % GPW;
+        tid = t / GPW;
+    } else {
+        tid = t % T;
+        g = t / T;
+    }
+
+    // Group -> (limb, group-in-limb).
+    const uint32_t gid = tile * GPW + g;
+    uint32_t limb, o;
+    if constexpr (MULTI) {
+        limb = gid / a.groups_per_limb;
+        o = gid % a.groups_per_limb;
+    } else {
+        limb = (tile * GPW) / a.groups_per_limb;  // uniform over the workgroup
+        o = gid - limb * a.groups_per_limb;
+    }
+    const bool valid = limb < a.num_limbs;
+    const uint32_t jl = valid ? limb : 0;
+    const NttJobDev J = load_job(a, jl);
+    const uint64_t q = J.q;
+    const uint64_t q2 = q << 1;
+    const uint64_t *src = a.first_pass ? J.src : J.dst;
+    uint64_t *dst = J.dst;
+    const ulonglong2 *tw = J.tw;
+    const uint32_t logn = a.logn;
+    const uint32_t n = 1u << logn;
+    const uint32_t blo = COL ? logn - LOG_G : 0u;
+
+    // Word offset of local index k of this thread's group.
+    auto gaddr = [&](uint32_t k) -> uint32_t {
+        if constexpr (COL) return (k << blo) | o;
+        else return (o << LOG_G) | k;
+    };
+
+    uint64_t v[E];
+
+    // ---- load: registers in chunk-0 layout
+    constexpr bool LOAD_VIA_LDS = !COL && !FWD;  // inverse ROW: chunk 0 is thread-contiguous
+    if constexpr (LOAD_VIA_LDS) {
+        // coalesced copy of the tile into LDS
+        for (uint32_t e = t; e < (uint32_t)(GPW * G); e += NT) {
+            const uint32_t ge = e >> LOG_G, ke = e & (G - 1);
+            const uint32_t gide = tile * GPW + ge;
+            uint32_t le, oe;
+            if constexpr (MULTI) {
+                le = gide / a.groups_per_limb;
+                oe = gide % a.groups_per_limb;
+            } else {
+                le = limb;
+                oe = gide - limb * a.groups_per_limb;
+            }
+            if (le < a.num_limbs) {
+                const uint64_t *se = MULTI ? job_src(a, le) : src;
+                lds[L::idx(ge, ke)] = se[(oe << LOG_G) | ke];
+            }
+        }
+        __syncthreads();
+        const uint32_t k0 = Gm::kt(0, tid);
+        static_for<0, E>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            v[r] = lds[L::idx(g, k0 + Gm::kr(0, r))];
+        });
+    } else {
+        const uint32_t k0 = Gm::kt(0, tid);
+        if (valid) {
+            static_for<0, E>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                v[r] = src[gaddr(k0 + Gm::kr(0, r))];
+            });
+        } else {
+            static_for<0, E>([&](auto rc) { v[decltype(rc)::value] = 0; });
+        }
+    }
+
+    // ---- chunks
+    static_for<0, NC>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        constexpr int B = Gm::B(c);
+        constexpr int LO = Gm::LO(c);
+        const uint32_t ktid = Gm::kt(c, tid);
+
+        if constexpr (c > 0) {
+            // redistribute: registers in chunk c-1 layout -> LDS -> chunk c layout
+            const uint32_t kp = Gm::kt(c - 1, tid);
+            __syncthreads();
+            static_for<0, E>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                lds[L::idx(g, kp + Gm::kr(c - 1, r))] = v[r];
+            });
+            __syncthreads();
+            static_for<0, E>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                v[r] = lds[L::idx(g, ktid + Gm::kr(c, r))];
+            });
+        }
+
+        // butterflies on local bits LO..LO+B-1 (forward: descending)
+        static_for<0, B>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            constexpr int p = FWD ? LO + B - 1 - s : LO + s;
+            constexpr int bit = 1 << (p - LO);
+            const uint32_t gb = p + blo;  // global bit
+            // thread part of P >> (gb+1); the register part is added per pair
+            uint32_t pt;
+            if constexpr (COL) pt = ktid >> (p + 1);
+            else pt = ((o << LOG_G) | ktid) >> (p + 1);
+            const bool fin = !FWD && a.final_inv && (p == LOG_G - 1);
+            uint64_t w[E], wq[E];
+            static_for<0, E>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                if constexpr ((r & bit) == 0) {
+                    constexpr int rp = Gm::rep(c, r, p);
+                    constexpr uint32_t pr = Gm::kr(c, r) >> (p + 1);
+                    if constexpr (rp == r) {
+                        if (!fin) {
+                            uint32_t ti;
+                            if constexpr (FWD) ti = (n >> (gb + 1)) + pt + pr;
+                            else ti = n + 1u - (n >> gb) + pt + pr;
+                            const ulonglong2 tv = tw[ti];
+                            w[r] = tv.x;
+                            wq[r] = tv.y;
+                        }
+                    }
+                    if constexpr (FWD) {
+                        fwd_bfly(v[r], v[r | bit], w[rp], wq[rp], q, q2);
+                    } else {
+                        if (fin)
+                            inv_final_bfly(v[r], v[r | bit], J.ninv, J.ninv_q, J.sroot, J.sroot_q, q, q2);
+                        else
+                            inv_bfly(v[r], v[r | bit], w[rp], wq[rp], q, q2);
+                    }
+                }
+            });
+        });
+    });
+
+    // ---- normalisation folded into the last pass (ntt.cpp:79-118)
+    if (a.normalize) {
+        static_for<0, E>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            if constexpr (FWD) v[r] = csub(csub(v[r], q2), q);
+            else v[r] = csub(v[r], q);
+        });
+    }
+
+    // ---- store
+    constexpr bool STORE_VIA_LDS = !COL && FWD;  // forward ROW: last chunk is thread-contiguous
+    if constexpr (STORE_VIA_LDS) {
+        const uint32_t kl = Gm::kt(NC - 1, tid);
+        __syncthreads();
+        static_for<0, E>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            lds[L::idx(g, kl + Gm::kr(NC - 1, r))] = v[r];
+        });
+        __syncthreads();
+        for (uint32_t e = t; e < (uint32_t)(GPW * G); e += NT) {
+            const uint32_t ge = e >> LOG_G, ke = e & (G - 1);
+            const uint32_t gide = tile * GPW + ge;
+            uint32_t le, oe;
+            if constexpr (MULTI) {
+                le = gide / a.groups_per_limb;
+                oe = gide % a.groups_per_limb;
+            } else {
+                le = limb;
+                oe = gide - limb * a.groups_per_limb;
+            }
+            if (le < a.num_limbs) {
+                uint64_t *de = MULTI ? job_dst(a, le) : dst;
+                de[(oe << LOG_G) | ke] = lds[L::idx(ge, ke)];
+            }
+        }
+    } else {
+        if (valid) {
+            const uint32_t kl = Gm::kt(NC - 1, tid);
+            static_for<0, E>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                dst[gaddr(kl + Gm::kr(NC - 1, r))] = v[r];
+            });
+        }
+    }
+}
+
+}  // namespace axhel_dev

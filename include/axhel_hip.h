@@ -1,0 +1,184 @@
+/*
+ * axhel_hip.h -- C ABI of the MI355X (gfx950) AXHEL library.
+ *
+ * Plain C: pointers, sizes and status codes only.  Streams are passed as
+ * `void *` holding a hipStream_t (NULL = the default stream).
+ *
+ * Pointer rule for every compute entry point: operands and results may be
+ * device pointers (hipMalloc / managed memory), which run asynchronously on
+ * `stream`, or ordinary host pointers, which the library stages through
+ * device memory and completes before returning.  Results may alias operands
+ * exactly, as in the reference (tests/public/test_eltwise.cpp:439-516).
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * massidonati/AXHEL).  Arithmetic is bit-identical to the reference's Native
+ * backend, lazy outputs included.
+ */
+#ifndef AXHEL_HIP_H
+#define AXHEL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum axhel_hip_status {
+    AXHEL_HIP_OK = 0,
+    /* Factor combination the reference does nothing for (silent no-op there:
+       eltwise-mul-mod.cpp:83-93, eltwise-fma-mod.cpp:98-111,
+       eltwise-reduce-mod.cpp:32-128).  Output is left untouched. */
+    AXHEL_HIP_UNSUPPORTED_FACTOR = 1,
+    AXHEL_HIP_INVALID_ARGUMENT = 2,
+    AXHEL_HIP_NO_MEMORY = 3,
+    AXHEL_HIP_HIP_ERROR = 4,
+    AXHEL_HIP_NO_DEVICE = 5,
+    AXHEL_HIP_REJECTED = 6,  /* serving: admission refused (memory budget / shutdown) */
+    AXHEL_HIP_TIMEOUT = 7,
+    AXHEL_HIP_CANCELLED = 8
+} axhel_hip_status;
+
+/* Human-readable text of a status, and the last HIP error string seen by the
+   calling thread (empty if none). */
+const char *axhel_hip_status_string(axhel_hip_status s);
+const char *axhel_hip_last_error(void);
+
+/* ------------------------------------------------------------------ device */
+
+/* Number of visible GPUs; 0 when no GPU is present (no error). */
+int axhel_hip_device_count(void);
+/* 1 if p is device-accessible memory (hipMalloc/managed), else 0. */
+int axhel_hip_is_device_pointer(const void *p);
+axhel_hip_status axhel_hip_stream_synchronize(void *stream);
+
+/* ------------------------------------------------------------- element-wise */
+
+/* EltwiseAddMod vector-vector: include/axhel/eltwise/eltwise-add-mod.hpp:13
+   (native body eltwise-add-mod.cpp:15-30). */
+axhel_hip_status axhel_hip_add_mod_vv(uint64_t *result, const uint64_t *operand1, const uint64_t *operand2,
+                                      uint64_t n, uint64_t modulus, void *stream);
+/* EltwiseAddMod vector-scalar: eltwise-add-mod.hpp:17 (eltwise-add-mod.cpp:33-47). */
+axhel_hip_status axhel_hip_add_mod_vs(uint64_t *result, const uint64_t *operand1, uint64_t operand2, uint64_t n,
+                                      uint64_t modulus, void *stream);
+/* EltwiseSubMod vector-vector: eltwise-sub-mod.hpp:13 (eltwise-sub-mod.cpp:15-30). */
+axhel_hip_status axhel_hip_sub_mod_vv(uint64_t *result, const uint64_t *operand1, const uint64_t *operand2,
+                                      uint64_t n, uint64_t modulus, void *stream);
+/* EltwiseSubMod vector-scalar: eltwise-sub-mod.hpp:17 (eltwise-sub-mod.cpp:33-47). */
+axhel_hip_status axhel_hip_sub_mod_vs(uint64_t *result, const uint64_t *operand1, uint64_t operand2, uint64_t n,
+                                      uint64_t modulus, void *stream);
+/* EltwiseMulMod: eltwise-mul-mod.hpp:14 (eltwise-mul-mod.cpp:19-94).
+   input_mod_factor outside {1,2,4} -> AXHEL_HIP_UNSUPPORTED_FACTOR, no write. */
+axhel_hip_status axhel_hip_mul_mod(uint64_t *result, const uint64_t *operand1, const uint64_t *operand2, uint64_t n,
+                                   uint64_t modulus, uint64_t input_mod_factor, void *stream);
+/* EltwiseFMAMod: eltwise-fma-mod.hpp:15 (eltwise-fma-mod.cpp:19-112).
+   operand3 may be NULL.  Factor outside {1,2,4,8} -> UNSUPPORTED_FACTOR. */
+axhel_hip_status axhel_hip_fma_mod(uint64_t *result, const uint64_t *operand1, uint64_t operand2,
+                                   const uint64_t *operand3, uint64_t n, uint64_t modulus, uint64_t input_mod_factor,
+                                   void *stream);
+/* EltwiseReduceMod: eltwise-reduce-mod.hpp:14 (eltwise-reduce-mod.cpp:32-140).
+   input_mod_factor == modulus selects the Barrett path for any 64-bit input.
+   Combinations the reference skips -> UNSUPPORTED_FACTOR, no write. */
+axhel_hip_status axhel_hip_reduce_mod(uint64_t *result, const uint64_t *operand, uint64_t n, uint64_t modulus,
+                                      uint64_t input_mod_factor, uint64_t output_mod_factor, void *stream);
+
+/* -------------------------------------------------------------------- NTT */
+
+/* An NTT plan: one (degree, modulus, root) with its twiddle tables resident
+   in device memory.  Immutable after creation; safe to share across threads. */
+typedef struct axhel_hip_plan axhel_hip_plan;
+
+/* NTT::NTT(degree, modulus, root_of_unity): include/axhel/ntt/ntt.hpp:32
+   (ntt.cpp:136-152, tables ntt-tables.cpp:81-140).  degree must be a power of
+   two in [2, 2^17] with modulus == 1 mod 2*degree, modulus < 2^62. */
+axhel_hip_status axhel_hip_ntt_plan_create(uint64_t degree, uint64_t modulus, uint64_t root_of_unity,
+                                           axhel_hip_plan **plan);
+/* NTT::NTT(degree, modulus, root_powers, inv_root_powers, inv_degree_modulo):
+   ntt.hpp:37 (ntt.cpp:124-133).  Tables are host arrays of `degree`
+   {operand, quotient} pairs (2*degree words, the NTTMultiplyOperand layout of
+   ntt.hpp:14-18); they are copied to the device once. */
+axhel_hip_status axhel_hip_ntt_plan_create_external(uint64_t degree, uint64_t modulus, const uint64_t *root_powers,
+                                                    const uint64_t *inv_root_powers, uint64_t inv_degree_operand,
+                                                    uint64_t inv_degree_quotient, axhel_hip_plan **plan);
+axhel_hip_status axhel_hip_ntt_plan_destroy(axhel_hip_plan *plan);
+/* NTT::Degree / NTT::Modulus: ntt.hpp:58-61 (ntt.cpp:210-216). */
+uint64_t axhel_hip_ntt_plan_degree(const axhel_hip_plan *plan);
+uint64_t axhel_hip_ntt_plan_modulus(const axhel_hip_plan *plan);
+/* Copies the plan's host-side tables out (2*degree words each, plus
+   {n^-1, quotient}); any pointer may be NULL. */
+axhel_hip_status axhel_hip_ntt_plan_tables(const axhel_hip_plan *plan, uint64_t *root_powers,
+                                           uint64_t *inv_root_powers, uint64_t *inv_degree);
+
+/* NTT::ComputeForward: ntt.hpp:49 (ntt.cpp:173-186).  input_mod_factor is
+   ignored as in the reference; output_mod_factor == 4 gives the lazy [0,4q)
+   output, anything else the normalized [0,q) output. */
+axhel_hip_status axhel_hip_ntt_forward(const axhel_hip_plan *plan, uint64_t *result, const uint64_t *operand,
+                                       uint64_t input_mod_factor, uint64_t output_mod_factor, void *stream);
+/* NTT::ComputeInverse: ntt.hpp:53 (ntt.cpp:192-204).  output_mod_factor == 2
+   gives the lazy [0,2q) output, anything else [0,q). */
+axhel_hip_status axhel_hip_ntt_inverse(const axhel_hip_plan *plan, uint64_t *result, const uint64_t *operand,
+                                       uint64_t input_mod_factor, uint64_t output_mod_factor, void *stream);
+
+/* ---------------------------------------------------------------- batching */
+
+/* One limb-transform of a batch: what one ComputeForward/ComputeInverse call
+   does, for a device-resident operand.  No reference counterpart: the
+   reference handles one vector per call and leaves the limb loop to the
+   caller (integration/seal/SEAL-4.4.5-AXHEL.patch:185-226). */
+typedef struct axhel_hip_ntt_job {
+    const axhel_hip_plan *plan;
+    uint64_t *result;
+    const uint64_t *operand;
+    uint64_t output_mod_factor;
+    int32_t inverse; /* 0 = ComputeForward, 1 = ComputeInverse */
+    int32_t reserved;
+} axhel_hip_ntt_job;
+
+typedef struct axhel_hip_batch axhel_hip_batch;
+
+/* Builds a reusable batch: jobs are grouped by (degree, direction), the job
+   tables are uploaded to device memory, a kernel configuration is picked per
+   degree (the autotuned table when one is loaded) and the launches are
+   captured into one hipGraph.  Every job pointer must be device memory and no
+   two jobs may write overlapping memory. */
+axhel_hip_status axhel_hip_ntt_batch_create(const axhel_hip_ntt_job *jobs, size_t count, axhel_hip_batch **batch);
+/* Replays the captured graph on `stream` (asynchronous). */
+axhel_hip_status axhel_hip_ntt_batch_launch(axhel_hip_batch *batch, void *stream);
+/* Same work as launch, issued as plain kernel launches without the graph. */
+axhel_hip_status axhel_hip_ntt_batch_launch_eager(axhel_hip_batch *batch, void *stream);
+axhel_hip_status axhel_hip_ntt_batch_destroy(axhel_hip_batch *batch);
+/* Number of kernel launches one replay issues, and the index-th of them run
+   alone on `stream`, in replay order (per-kernel timing with events). */
+int axhel_hip_ntt_batch_kernel_count(const axhel_hip_batch *batch);
+axhel_hip_status axhel_hip_ntt_batch_launch_kernel(axhel_hip_batch *batch, int index, void *stream);
+/* One-shot batch: create, launch eagerly on `stream`, synchronize, destroy. */
+axhel_hip_status axhel_hip_ntt_batch_run(const axhel_hip_ntt_job *jobs, size_t count, void *stream);
+
+/* ---------------------------------------------------------------- tuning */
+
+/* Kernel configuration for one degree.  A transform runs as one ROW pass
+   (all index bits in one workgroup's LDS) or as a COL pass over the top
+   log_col bits followed by a ROW pass over the rest (inverse: reversed).
+   log_elems = log2 words per thread in the ROW pass, i.e. the butterfly
+   stages done in registers between LDS exchanges (the radix); row_groups /
+   col_groups = groups (rows / columns) per workgroup. */
+typedef struct axhel_hip_ntt_config {
+    int32_t log_elems;
+    int32_t row_groups;
+    int32_t col_groups; /* 0 for a single pass */
+    int32_t log_col;    /* 0 for a single pass */
+} axhel_hip_ntt_config;
+
+/* Number of compiled configurations for a degree, and the i-th of them. */
+int axhel_hip_ntt_config_count(uint64_t degree);
+axhel_hip_status axhel_hip_ntt_config_get(uint64_t degree, int index, axhel_hip_ntt_config *config);
+/* Selects the configuration used for a degree from now on (process-wide);
+   index -1 restores the built-in default. */
+axhel_hip_status axhel_hip_ntt_config_select(uint64_t degree, int index);
+int axhel_hip_ntt_config_selected(uint64_t degree);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AXHEL_HIP_H */

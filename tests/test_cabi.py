@@ -1,0 +1,74 @@
+"""The C-ABI libraries load and export what the headers declare (CPU only, no
+compute calls)."""
+import os
+import re
+import subprocess
+
+import axhel_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_c_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(axhel_hip_\w+)\s*\(", text))
+    return names
+
+
+def exported(path, demangle=False):
+    args = ["nm", "-D", "--defined-only"] + (["-C"] if demangle else []) + [path]
+    return subprocess.run(args, capture_output=True, text=True, check=True).stdout
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    lib = axhel_amd.lib()
+    names = declared_c_functions()
+    assert len(names) >= 25
+    syms = exported(axhel_amd.LIB_PATH)
+    for name in sorted(names):
+        assert re.search(rf"\bT {name}$", syms, flags=re.M), name
+        assert hasattr(lib, name)
+
+
+def test_python_binding_covers_the_header():
+    assert declared_c_functions() == set(axhel_amd.SIGNATURES)
+
+
+def test_cxx_dropin_exports_reference_api():
+    syms = exported(axhel_amd.CXX_LIB_PATH, demangle=True)
+    for sig in [
+        "unipi::axhel::EltwiseAddMod(unsigned long*, unsigned long const*, unsigned long const*, unsigned long, unsigned long)",
+        "unipi::axhel::EltwiseAddMod(unsigned long*, unsigned long const*, unsigned long, unsigned long, unsigned long)",
+        "unipi::axhel::EltwiseSubMod(unsigned long*, unsigned long const*, unsigned long const*, unsigned long, unsigned long)",
+        "unipi::axhel::EltwiseSubMod(unsigned long*, unsigned long const*, unsigned long, unsigned long, unsigned long)",
+        "unipi::axhel::EltwiseMulMod(unsigned long*, unsigned long const*, unsigned long const*, unsigned long, unsigned long, unsigned long)",
+        "unipi::axhel::EltwiseFMAMod(unsigned long*, unsigned long const*, unsigned long, unsigned long const*, unsigned long, unsigned long, unsigned long)",
+        "unipi::axhel::EltwiseReduceMod(unsigned long*, unsigned long const*, unsigned long, unsigned long, unsigned long, unsigned long)",
+        "unipi::axhel::NTT::NTT(unsigned long, unsigned long, unsigned long)",
+        "unipi::axhel::NTT::NTT(unsigned long, unsigned long, unipi::axhel::NTTMultiplyOperand const*, unipi::axhel::NTTMultiplyOperand const*, unipi::axhel::NTTMultiplyOperand)",
+        "unipi::axhel::NTT::ComputeForward(unsigned long*, unsigned long const*, unsigned long, unsigned long) const",
+        "unipi::axhel::NTT::ComputeInverse(unsigned long*, unsigned long const*, unsigned long, unsigned long) const",
+        "unipi::axhel::NTT::Degree() const",
+        "unipi::axhel::NTT::Modulus() const",
+    ]:
+        assert sig in syms, sig
+
+
+def test_kernels_are_built_for_gfx950():
+    data = open(axhel_amd.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in data
+
+
+def test_configs_are_listed_without_a_gpu():
+    for logn in range(1, 18):
+        n = 1 << logn
+        assert axhel_amd.config_count(n) >= 1
+        c = axhel_amd.config_get(n, 0)
+        assert c["log_elems"] in (1, 2, 3, 4)
+    assert axhel_amd.config_count(3) == 0
+    assert axhel_amd.config_count(1 << 18) == 0

@@ -1,0 +1,108 @@
+"""Batched NTT (many polynomials x RNS limbs, uneven degrees, one hipGraph)
+bit-exact against the oracle, job by job."""
+import numpy as np
+import pytest
+
+import axhel_amd as ax
+from axhel_amd import params
+from oracle import oracle as O
+from tests.helpers import dev, empty_dev, host, torch_mod
+
+pytestmark = pytest.mark.gpu
+
+
+def test_rns_batch_n65536_24_limbs_all_modes():
+    """The 64 x 24 x 2^16 workload's shape at 4 polynomials: every limb of every
+    polynomial, forward and inverse, lazy and normalized."""
+    torch = torch_mod()
+    n, L, P = 1 << 16, 24, 4
+    primes = params.rns_primes(n, L)
+    psis = [params.find_primitive_2n_root(n, q) for q in primes]
+    plans = [ax.NTT(n, q, psi) for q, psi in zip(primes, psis)]
+    tabs = [O.Tables(n, q, psi) for q, psi in zip(primes, psis)]
+    rng = np.random.default_rng(1)
+    host_in = np.stack([np.stack([params.uniform(rng, n, 4 * q) for q in primes]) for _ in range(P)])
+    for inverse in (False, True):
+        for lazy in (True, False):
+            data = host_in if not inverse else np.stack(
+                [np.stack([host_in[p, l] % (2 * primes[l]) for l in range(L)]) for p in range(P)])
+            s = dev(data.reshape(-1)).view(P, L, n)
+            out = torch.empty_like(s)
+            of = (2 if inverse else 4) if lazy else 1
+            jobs = [(plans[l], out[p, l], s[p, l], inverse, of) for p in range(P) for l in range(L)]
+            b = ax.Batch(jobs)
+            b.launch()
+            got = host(out).reshape(P, L, n)
+            for p in range(P):
+                for l in range(L):
+                    if inverse:
+                        want = O.compute_inverse(data[p, l], tabs[l], 2, of)
+                    else:
+                        want = O.compute_forward(data[p, l], tabs[l], 4, of)
+                    assert np.array_equal(got[p, l], want), (inverse, lazy, p, l)
+            b.close()
+
+
+def test_uneven_batch_graph_replays():
+    """512-job style mix scaled down: N in 2^12..2^17, 1..6 limbs per job,
+    30..60-bit moduli, both directions and factors; the captured graph is
+    replayed three times with eager work in between, each replay checked."""
+    torch = torch_mod()
+    rng = np.random.default_rng(2024)
+    jobs, refs, keep = [], [], []
+    plan_cache = {}
+    for j in range(40):
+        n = 1 << int(rng.integers(12, 18))
+        limbs = int(rng.integers(1, 7))
+        inverse = bool(rng.integers(0, 2))
+        lazy = bool(rng.integers(0, 2))
+        of = (2 if inverse else 4) if lazy else 1
+        for l in range(limbs):
+            bits = int(rng.integers(30, 61))
+            q = params.ntt_primes(n, bits, 1)[0]
+            key = (n, q)
+            if key not in plan_cache:
+                psi = params.find_primitive_2n_root(n, q)
+                plan_cache[key] = (ax.NTT(n, q, psi), O.Tables(n, q, psi))
+            plan, tab = plan_cache[key]
+            a = params.uniform(rng, n, (2 if inverse else 4) * q)
+            s, o = dev(a), empty_dev(n)
+            keep += [s, o]
+            jobs.append((plan, o, s, inverse, of))
+            want = O.compute_inverse(a, tab, 2, of) if inverse else O.compute_forward(a, tab, 4, of)
+            refs.append((o, want))
+    b = ax.Batch(jobs)
+    for rep in range(3):
+        for o, _ in refs:
+            o.fill_(-1)
+        b.launch()
+        torch.cuda.synchronize()
+        for i, (o, want) in enumerate(refs):
+            assert np.array_equal(host(o), want), (rep, i)
+        x = torch.arange(1 << 20, device="cuda").sum()  # unrelated eager work between replays
+        torch.cuda.synchronize()
+        assert int(x) == (1 << 20) * ((1 << 20) - 1) // 2
+    b.launch_eager()
+    for i, (o, want) in enumerate(refs):
+        assert np.array_equal(host(o), want), ("eager", i)
+    b.close()
+
+
+def test_in_place_batch_jobs():
+    n = 1 << 15
+    qs = params.rns_primes(n, 8)
+    rng = np.random.default_rng(3)
+    jobs, refs = [], []
+    keep = []
+    for q in qs:
+        psi = params.find_primitive_2n_root(n, q)
+        plan, tab = ax.NTT(n, q, psi), O.Tables(n, q, psi)
+        a = params.uniform(rng, n, q)
+        d = dev(a)
+        keep.append(plan)
+        jobs.append((plan, d, d, False, 1))
+        refs.append((d, O.compute_forward(a, tab, 1, 1)))
+    b = ax.Batch(jobs)
+    b.launch()
+    for d, want in refs:
+        assert np.array_equal(host(d), want)

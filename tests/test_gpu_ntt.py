@@ -1,0 +1,148 @@
+"""Plain per-call NTT (ComputeForward / ComputeInverse drop-ins) on the GPU,
+bit-exact against the oracle, lazy outputs included (tests/public/test_ntt.cpp
+restated, extended to every compiled kernel configuration and N up to 2^17)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import axhel_amd as ax
+from oracle import oracle as O
+from tests.helpers import NTT_DEGREES, MODULUS_BITS, SEED, dev, empty_dev, host, ntt_params
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+
+
+def test_kat_n8_q17():
+    k = GOLD["ntt_n8_q17"]
+    ntt = ax.NTT(k["n"], k["q"], k["psi"])
+    fwd, inv, ni = ntt.tables()
+    assert fwd.reshape(-1, 2).tolist() == k["root_powers"]
+    assert inv.reshape(-1, 2).tolist() == k["inv_root_powers"]
+    assert ni.tolist() == k["inv_degree_modulo"]
+    x = dev(k["input"])
+    ntt.ComputeForward(x, x, 1, 1)
+    assert host(x).tolist() == k["forward_normalized"]
+    ntt.ComputeInverse(x, x, 1, 1)
+    assert host(x).tolist() == k["input"]
+
+
+def _check_all_modes(ntt, t, n, q, rng, label):
+    a4 = O.MT19937_64(rng).random_vector(n, 4 * q)  # forward accepts [0, 4q)
+    a2 = O.MT19937_64(rng + 1).random_vector(n, 2 * q)  # inverse accepts [0, 2q)
+    d = dev(a4)
+    r = empty_dev(n)
+    ntt.ComputeForward(r, d, 4, 4)
+    assert np.array_equal(host(r), O.compute_forward(a4, t, 4, 4)), ("fwd lazy", label)
+    ntt.ComputeForward(r, d, 4, 1)
+    assert np.array_equal(host(r), O.compute_forward(a4, t, 4, 1)), ("fwd norm", label)
+    d = dev(a2)
+    ntt.ComputeInverse(r, d, 2, 2)
+    assert np.array_equal(host(r), O.compute_inverse(a2, t, 2, 2)), ("inv lazy", label)
+    ntt.ComputeInverse(r, d, 2, 1)
+    assert np.array_equal(host(r), O.compute_inverse(a2, t, 2, 1)), ("inv norm", label)
+
+
+@pytest.mark.parametrize("logn", list(range(1, 18)))
+def test_every_config_bit_exact(logn):
+    n = 1 << logn
+    try:
+        for bits in (30, 50, 62):
+            q, psi = ntt_params(n, bits)
+            ntt = ax.NTT(n, q, psi)
+            t = O.Tables(n, q, psi)
+            for ci in range(ax.config_count(n)):
+                ax.config_select(n, ci)
+                _check_all_modes(ntt, t, n, q, SEED ^ (logn * 131 + bits + ci), (n, bits, ci))
+    finally:
+        ax.config_select(n, -1)
+
+
+@pytest.mark.parametrize("n", NTT_DEGREES)
+def test_round_trips_all_widths(n):
+    """test_ntt.cpp:124-162 on the default configuration, every modulus width."""
+    for bits in MODULUS_BITS:
+        q, psi = ntt_params(n, bits)
+        ntt = ax.NTT(n, q, psi)
+        a = O.MT19937_64(SEED ^ n ^ bits).random_vector(n, q)
+        x = dev(a)
+        ntt.ComputeForward(x, x, 1, 1)
+        ntt.ComputeInverse(x, x, 1, 1)
+        assert np.array_equal(host(x), a), (n, bits)
+        x = dev(a)
+        ntt.ComputeForward(x, x, 1, 4)
+        ax.EltwiseReduceMod(x, x, n, q, 4, 2)
+        ntt.ComputeInverse(x, x, 2, 1)
+        assert np.array_equal(host(x), a), ("lazy", n, bits)
+
+
+def test_out_of_place_matches_in_place_and_host_pointers():
+    for n in (8, 128, 1024, 8192, 32768, 65536, 131072):
+        q, psi = ntt_params(n, 62)
+        ntt = ax.NTT(n, q, psi)
+        a = O.MT19937_64(SEED ^ n).random_vector(n, q)
+        ip = dev(a)
+        ntt.ComputeForward(ip, ip, 1, 1)
+        src = dev(a)
+        out = empty_dev(n)
+        ntt.ComputeForward(out, src, 1, 1)
+        assert np.array_equal(host(out), host(ip))
+        assert np.array_equal(host(src), a)  # operand untouched out of place
+        h = a.copy()
+        ntt.ComputeForward(h, h, 1, 1)  # host buffer, staged
+        assert np.array_equal(h, host(ip))
+        ho = np.zeros(n, dtype=np.uint64)
+        ntt.ComputeInverse(ho, h, 1, 1)
+        assert np.array_equal(ho, a)
+
+
+def test_external_tables_constructor():
+    k = GOLD["ntt_n8_q17"]
+    ext = ax.NTT(8, 17, root_powers=np.array(k["root_powers"], dtype=np.uint64).ravel(),
+                 inv_root_powers=np.array(k["inv_root_powers"], dtype=np.uint64).ravel(),
+                 inv_degree_modulo=k["inv_degree_modulo"])
+    x = dev(k["input"])
+    ext.ComputeForward(x, x, 1, 1)
+    assert host(x).tolist() == k["forward_normalized"]
+    ext.ComputeInverse(x, x, 1, 1)
+    assert host(x).tolist() == k["input"]
+    n = 4096
+    q, psi = ntt_params(n, 50)
+    t = O.Tables(n, q, psi)
+    ext = ax.NTT(n, q, root_powers=t.fwd, inv_root_powers=t.inv, inv_degree_modulo=t.ninv)
+    assert ext.Degree() == n and ext.Modulus() == q
+    a = O.MT19937_64(3).random_vector(n, 4 * q)
+    x = dev(a)
+    ext.ComputeForward(x, x, 1, 4)
+    assert np.array_equal(host(x), O.compute_forward(a, t, 1, 4))
+
+
+def test_special_patterns():
+    """test_ntt.cpp:15-30 patterns, lazy and normalized, against the oracle."""
+    for n in (8, 4096, 65536):
+        q, psi = ntt_params(n, 60)
+        ntt = ax.NTT(n, q, psi)
+        t = O.Tables(n, q, psi)
+        pats = [np.zeros(n, np.uint64), np.eye(1, n, 0, dtype=np.uint64)[0], np.ones(n, np.uint64),
+                np.full(n, q - 1, np.uint64), np.array([(q - 1) * (i & 1) for i in range(n)], np.uint64),
+                np.array([q - 1 if i == n - 1 else 0 for i in range(n)], np.uint64),
+                np.full(n, 4 * q - 1, np.uint64)]
+        for a in pats:
+            r = empty_dev(n)
+            ntt.ComputeForward(r, dev(a), 1, 4)
+            assert np.array_equal(host(r), O.compute_forward(a, t, 1, 4))
+            ntt.ComputeForward(r, dev(a), 1, 1)
+            assert np.array_equal(host(r), O.compute_forward(a, t, 1, 1))
+            b = np.minimum(a, 2 * q - 1)
+            ntt.ComputeInverse(r, dev(b), 2, 2)
+            assert np.array_equal(host(r), O.compute_inverse(b, t, 2, 2))
+
+
+def test_invalid_plans_are_rejected():
+    with pytest.raises(ax.AxhelError):
+        ax.NTT(12, 97, 3)  # not a power of two
+    with pytest.raises(ax.AxhelError):
+        ax.NTT(1 << 18, 0x3FFFFFFFFFF00001, 3)  # above the supported degree
