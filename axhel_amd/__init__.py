@@ -95,6 +95,7 @@ SIGNATURES = {
     "axhel_hip_ntt_config_get": (_st, [_u64, ctypes.c_int, ctypes.POINTER(NttConfig)]),
     "axhel_hip_ntt_config_select": (_st, [_u64, ctypes.c_int]),
     "axhel_hip_ntt_config_selected": (ctypes.c_int, [_u64]),
+    "axhel_hip_ubench_butterfly_peak": (_st, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
 }
 
 
@@ -312,6 +313,13 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def butterfly_peak():
+    """Device-wide rate of register-resident forward Harvey butterflies."""
+    r, s = ctypes.c_double(), ctypes.c_double()
+    _check(_load().axhel_hip_ubench_butterfly_peak(ctypes.byref(r), ctypes.byref(s)), "butterfly_peak")
+    return r.value, s.value
 
 
 # ------------------------------------------------------------------ tuning

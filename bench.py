@@ -62,6 +62,39 @@ def cpu_baseline(n, primes, psis, seconds):
             "sample": f"{total} forward NTTs at N={n} over {el:.1f} s, one limb per thread"}
 
 
+def eltwise_leg(ax, torch, src, dst, q, reps):
+    """Each element-wise op over the whole resident array (one call, one
+    modulus), timed with HIP events: algorithmic GB/s and fraction of HBM."""
+    flat_a = src.view(-1)
+    flat_r = dst.view(-1)
+    n = flat_a.numel()
+    half = n // 2
+    a, b, r = flat_a[:half], flat_a[half:2 * half], flat_r[:half]
+    st = torch.cuda.current_stream()
+    ops = {
+        "add_vv": (24, lambda: ax.EltwiseAddMod(r, a, b, half, q, stream=st)),
+        "sub_vv": (24, lambda: ax.EltwiseSubMod(r, a, b, half, q, stream=st)),
+        "mul_f1": (24, lambda: ax.EltwiseMulMod(r, a, b, half, q, 1, stream=st)),
+        "fma_f1": (24, lambda: ax.EltwiseFMAMod(r, a, 12345, b, half, q, 1, stream=st)),
+        "reduce_barrett": (16, lambda: ax.EltwiseReduceMod(r, a, half, q, q, 1, stream=st)),
+        "reduce_4_1": (16, lambda: ax.EltwiseReduceMod(r, a, half, q, 4, 1, stream=st)),
+    }
+    out = {}
+    st = torch.cuda.current_stream()
+    for name, (bpe, fn) in ops.items():
+        fn()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for e0, e1 in evs:
+            e0.record(st)
+            fn()
+            e1.record(st)
+        torch.cuda.synchronize()
+        ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / reps
+        gbs = bpe * half / (ms * 1e-3) / 1e9
+        out[name] = {"elements": half, "ms": ms, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +106,10 @@ def main():
     ap.add_argument("--plain-limbs", type=int, default=384, help="limbs timed on the plain per-call path")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-eltwise", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run one 2^27-word EltwiseReduceMod(4->2) (1 GiB in, 1 GiB out): "
+                         "the known-size stream that scales rocprof FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -84,6 +121,14 @@ def main():
     from axhel_amd import params
 
     assert torch.cuda.is_available(), "bench needs a GPU"
+    if args.calibrate_copy:
+        words = 1 << 27
+        a = torch.zeros(words, dtype=torch.int64, device="cuda")
+        b = torch.empty_like(a)
+        ax.EltwiseReduceMod(b, a, words, 0x0FFFFFFFFFFC0001, 4, 2)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {words} words read and written")
+        return
     n, L, P = 1 << args.logn, args.limbs, args.polys
     primes = params.rns_primes(n, L)
     psis = [params.find_primitive_2n_root(n, q) for q in primes]
@@ -145,14 +190,24 @@ def main():
             kdur.append(sum(a.elapsed_time(b) for a, b in evs) / reps)
     log("kernel avg ms:", [round(x, 4) for x in kdur])
     dom = int(np.argmax(kdur))
-    # algorithmic bytes of one pass: every coefficient read once and written once
+    # Algorithmic work of one pass launch: each of the two passes runs half of
+    # the log2(N) stages, (N/2)*(log2(N)/2) butterflies per limb, and reads and
+    # writes every coefficient once (16 N bytes per limb).
+    bfly = (n // 2) * (args.logn / 2.0) * limbs
     alg_bytes = 16.0 * n * limbs
-    achieved = alg_bytes / (kdur[dom] * 1e-3) / 1e9
+    achieved_bfly = bfly / (kdur[dom] * 1e-3) / 1e9
+    achieved_gbs = alg_bytes / (kdur[dom] * 1e-3) / 1e9
+    peak_bfly, _ = ax.butterfly_peak()
+    peak_bfly /= 1e9
     cfg = ax.config_get(n, ax.config_selected(n))
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "ntt_n65536_traffic.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(f"pass{dom}_bytes")
+    tpath = os.path.join(ROOT, "profiles", "ntt_n65536_64x24_hbm_bytes.txt")
+    if os.path.exists(tpath) and args.logn == 16 and P == 64 and L == 24:
+        for line in open(tpath):
+            if line.startswith("traffic_bytes"):
+                kv = dict(x.split("=") for x in line.split()[1:])
+                traffic = float(kv.get(f"pass{dom}", "nan"))
+    log(f"dominant pass {dom}: {achieved_bfly:.1f} Gbfly/s of {peak_bfly:.1f} peak; {achieved_gbs:.0f} GB/s")
 
     # plain per-call path: one synchronous ComputeForward per limb
     pl = min(args.plain_limbs, limbs)
@@ -166,6 +221,11 @@ def main():
         ax.synchronize()
     plain = pl / (time.perf_counter() - t0)
     log(f"plain per-call: {plain:.4g} limbs/s ({pl} limbs)")
+
+    elt = None
+    if not args.no_eltwise:
+        elt = eltwise_leg(ax, torch, src, dst, primes[0], reps=max(3, min(10, args.steps)))
+        log("eltwise GB/s:", {k: round(v["GB/s"], 1) for k, v in elt.items()})
 
     cpu = None
     if not args.no_cpu:
@@ -192,9 +252,13 @@ def main():
         "batched_vs_plain": value / plain,
         "inverse_limbs_per_s": inv_rate,
         "kernel_ms": kdur,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"ntt pass {dom} of {kcount}"},
+        "roofline": {"bound": "valu", "achieved": achieved_bfly, "peak": peak_bfly, "unit": "Gbfly/s",
+                     "frac": achieved_bfly / peak_bfly, "traffic": traffic,
+                     "kernel": f"ntt pass {dom} of {kcount} ({kdur[dom]:.4f} ms)",
+                     "peak_source": "register-resident Harvey butterflies, all CUs (axhel_hip_ubench_butterfly_peak)",
+                     "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved_gbs / HBM_PEAK_GBS}},
+        "eltwise": elt,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
