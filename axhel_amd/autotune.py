@@ -1,0 +1,88 @@
+"""Autotuner for the NTT kernel configurations.
+
+For each degree it times every compiled configuration (radix = words per
+thread, groups per workgroup, pass split) on a synthetic batch sized to fill
+the GPU, forward and inverse, and keeps the fastest.  The result is a small
+text table, one `log2(N) index` line per degree, that the library reads at
+start-up (`axhel_amd/ntt_tuning_gfx950.txt`, or the file named by
+AXHEL_HIP_TUNING).
+
+Usage: python -m axhel_amd.autotune [--logn 12 13 ...] [--out PATH] [--words W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import axhel_amd as ax
+from axhel_amd import params
+
+DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ntt_tuning_gfx950.txt")
+
+
+def time_batch(batch, torch, reps):
+    batch.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.launch()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def tune_degree(logn, torch, words, reps, limbs_per_prime=64):
+    n = 1 << logn
+    limbs = max(1, words // n)
+    nprimes = max(1, min(24, limbs // limbs_per_prime)) if limbs >= limbs_per_prime else 1
+    primes = params.rns_primes(n, nprimes)
+    plans = [ax.NTT(n, q, params.find_primitive_2n_root(n, q)) for q in primes]
+    g = torch.Generator(device="cuda").manual_seed(logn)
+    src = torch.empty((limbs, n), dtype=torch.int64, device="cuda")
+    for i in range(limbs):
+        src[i] = torch.randint(0, primes[i % nprimes], (n,), generator=g, device="cuda", dtype=torch.int64)
+    dst = torch.empty_like(src)
+    results = []
+    for ci in range(ax.config_count(n)):
+        ax.config_select(n, ci)
+        fb = ax.Batch([(plans[i % nprimes], dst[i], src[i], False, 1) for i in range(limbs)])
+        ib = ax.Batch([(plans[i % nprimes], src[i], dst[i], True, 1) for i in range(limbs)])
+        tf = time_batch(fb, torch, reps)
+        ti = time_batch(ib, torch, reps)
+        fb.close()
+        ib.close()
+        results.append({"index": ci, "config": ax.config_get(n, ci), "fwd_ms": tf * 1e3, "inv_ms": ti * 1e3,
+                        "limbs": limbs, "fwd_limbs_per_s": limbs / tf, "inv_limbs_per_s": limbs / ti})
+        print(json.dumps({"logn": logn, **results[-1]}), flush=True)
+    ax.config_select(n, -1)
+    best = min(results, key=lambda r: r["fwd_ms"] + r["inv_ms"])
+    return best["index"], results
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--logn", type=int, nargs="*", default=list(range(10, 18)))
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--words", type=int, default=1 << 26, help="coefficients per timed batch")
+    ap.add_argument("--reps", type=int, default=5)
+    args = ap.parse_args()
+    import torch
+    assert torch.cuda.is_available(), "autotune needs a GPU"
+    table = {}
+    if os.path.exists(args.out):
+        for line in open(args.out):
+            parts = line.split()
+            if len(parts) == 2 and not line.startswith("#"):
+                table[int(parts[0])] = int(parts[1])
+    for logn in args.logn:
+        best, _ = tune_degree(logn, torch, args.words, args.reps)
+        table[logn] = best
+    with open(args.out, "w") as f:
+        f.write("# log2(N) config-index, chosen by axhel_amd.autotune on gfx950\n")
+        for logn in sorted(table):
+            f.write(f"{logn} {table[logn]}\n")
+    print("wrote", args.out, table, file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()

@@ -53,7 +53,7 @@ def build(verbose=False, jobs=None):
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile_hip(s, verbose), sources))
     hip_lib = os.path.join(LIBDIR, "libaxhel_hip.so")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", hip_lib + ".tmp"] + objs + ["-lpthread"],
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", hip_lib + ".tmp"] + objs + ["-lpthread", "-ldl"],
          verbose)
     os.replace(hip_lib + ".tmp", hip_lib)
 

@@ -36,32 +36,67 @@ __device__ __forceinline__ uint64_t shoup(uint64_t y, uint64_t w, uint64_t wq, u
     return y * w - qhat * q;
 }
 
+// The same Shoup product, spelled for the gfx950 VALU.  nq = 2^64 - q, so
+// y*w - qhat*q == y*w + qhat*nq (mod 2^64); the two low products then chain
+// through v_mad_u64_u32 accumulators and the four cross terms land in the
+// high word with two 3-input adds, with no 64-bit carry chain.  Same value
+// as shoup() for every input.
+__device__ __forceinline__ uint64_t shoup_n(uint64_t y, uint64_t w, uint64_t wq, uint64_t nq) {
+    const uint64_t qhat = __umul64hi(y, wq);
+    const uint32_t yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
+    const uint32_t wl = (uint32_t)w, wh = (uint32_t)(w >> 32);
+    const uint32_t ql = (uint32_t)qhat, qh = (uint32_t)(qhat >> 32);
+    const uint32_t nl = (uint32_t)nq, nh = (uint32_t)(nq >> 32);
+    uint64_t t = (uint64_t)yl * wl + (uint64_t)ql * nl;
+    const uint32_t cross = yl * wh + yh * wl + ql * nh + qh * nl;
+    t += (uint64_t)cross << 32;
+    return t;
+}
+
+// Per-limb constants of the butterflies: q, 2q and their negations mod 2^64.
+struct ModConsts {
+    uint64_t q, q2, nq, nq2;
+};
+
+__device__ __forceinline__ ModConsts mod_consts(uint64_t q) {
+    ModConsts m;
+    m.q = q;
+    m.q2 = q << 1;
+    m.nq = 0 - q;
+    m.nq2 = 0 - m.q2;
+    return m;
+}
+
+// csub(x, b) given nb = -b: one 64-bit add (v_lshl_add_u64) and a select.
+__device__ __forceinline__ uint64_t csub_n(uint64_t x, uint64_t b, uint64_t nb) {
+    const uint64_t d = x + nb;
+    return x >= b ? d : x;
+}
+
 // Harvey forward (Cooley-Tukey) butterfly, ntt-internal.hpp:20-31.
-__device__ __forceinline__ void fwd_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wq, uint64_t q,
-                                         uint64_t q2) {
-    const uint64_t xh = csub(x, q2);
-    const uint64_t t = shoup(y, w, wq, q);
+__device__ __forceinline__ void fwd_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wq, const ModConsts &m) {
+    const uint64_t xh = csub_n(x, m.q2, m.nq2);
+    const uint64_t t = shoup_n(y, w, wq, m.nq);
     x = xh + t;
-    y = xh + q2 - t;
+    y = xh + m.q2 - t;
 }
 
 // Harvey inverse (Gentleman-Sande) butterfly, ntt-internal.hpp:37-49.
-__device__ __forceinline__ void inv_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wq, uint64_t q,
-                                         uint64_t q2) {
+__device__ __forceinline__ void inv_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wq, const ModConsts &m) {
     const uint64_t u = x, v = y;
-    x = csub(u + v, q2);
-    y = shoup(u + q2 - v, w, wq, q);
+    x = csub_n(u + v, m.q2, m.nq2);
+    y = shoup_n(u + m.q2 - v, w, wq, m.nq);
 }
 
 // Final inverse stage butterfly with n^-1 and the scaled root folded in,
 // ntt-native.cpp:126-141.
 __device__ __forceinline__ void inv_final_bfly(uint64_t &x, uint64_t &y, uint64_t ni, uint64_t niq, uint64_t s,
-                                               uint64_t sq, uint64_t q, uint64_t q2) {
-    const uint64_t xh = csub(x, q2);
+                                               uint64_t sq, const ModConsts &m) {
+    const uint64_t xh = csub_n(x, m.q2, m.nq2);
     const uint64_t sum = xh + y;
-    const uint64_t dif = xh + q2 - y;
-    x = shoup(sum, ni, niq, q);
-    y = shoup(dif, s, sq, q);
+    const uint64_t dif = xh + m.q2 - y;
+    x = shoup_n(sum, ni, niq, m.nq);
+    y = shoup_n(dif, s, sq, m.nq);
 }
 
 // Barrett constants of eltwise-mul-mod.cpp:22-30 (n = BitWidth(q),

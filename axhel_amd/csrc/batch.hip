@@ -28,6 +28,7 @@ struct axhel_hip_batch {
         bool normalize;
         uint32_t count;
         size_t offset;  // first descriptor in d_jobs
+        bool vec16;     // all src/dst 16-byte aligned
     };
     std::vector<Group> groups;
     NttJobDev *d_jobs = nullptr;
@@ -55,7 +56,7 @@ hipError_t issue(axhel_hip_batch *b, hipStream_t stream) {
     static const NttJobDev none{};
     for (const auto &g : b->groups) {
         hipError_t e = launch_ntt(config_for(g.logn), g.logn, g.inverse, g.normalize, b->d_jobs + g.offset, none,
-                                  g.count, stream);
+                                  g.count, stream, -1, g.vec16);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -86,7 +87,7 @@ hipError_t capture(axhel_hip_batch *b) {
         for (size_t gi = 0; gi < b->groups.size() && ce == hipSuccess; ++gi) {
             const auto &g = b->groups[gi];
             ce = launch_ntt(config_for(g.logn), g.logn, g.inverse, g.normalize, b->d_jobs + g.offset, none, g.count,
-                            b->side[gi % nb]);
+                            b->side[gi % nb], -1, g.vec16);
         }
         for (int i = 0; i < nb && ce == hipSuccess; ++i) {
             ce = hipEventRecord(b->events[i], b->side[i]);
@@ -140,7 +141,11 @@ axhel_hip_status axhel_hip_ntt_batch_create(const axhel_hip_ntt_job *jobs, size_
         g.normalize = std::get<2>(kv.first) != 0;
         g.count = (uint32_t)idx.size();
         g.offset = host.size();
-        for (size_t i : idx) host.push_back(make_job(jobs[i].plan, jobs[i].operand, jobs[i].result, g.inverse));
+        g.vec16 = true;
+        for (size_t i : idx) {
+            host.push_back(make_job(jobs[i].plan, jobs[i].operand, jobs[i].result, g.inverse));
+            g.vec16 = g.vec16 && aligned16(jobs[i].operand) && aligned16(jobs[i].result);
+        }
         b->groups.push_back(g);
     }
     // Largest groups first: they fill the device while smaller ones trail.
@@ -192,7 +197,7 @@ axhel_hip_status axhel_hip_ntt_batch_launch_kernel(axhel_hip_batch *batch, int i
         const NttConfig &cfg = config_for(g.logn);
         if (index < cfg.npass) {
             AXHEL_HIP_TRY(launch_ntt(cfg, g.logn, g.inverse, g.normalize, batch->d_jobs + g.offset, none, g.count,
-                                     as_stream(stream), index));
+                                     as_stream(stream), index, g.vec16));
             return AXHEL_HIP_OK;
         }
         index -= cfg.npass;

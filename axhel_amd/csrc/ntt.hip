@@ -2,10 +2,15 @@
 // C-ABI entry points (ComputeForward / ComputeInverse drop-ins).
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <atomic>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <string>
 
 #include "ntt_host.hpp"
 
@@ -136,10 +141,43 @@ std::once_flag g_reg_once;
 std::vector<std::vector<NttConfig>> *g_reg = nullptr;
 std::atomic<int> g_selected[kMaxLogN + 1];
 
+// Tuned configuration table written by axhel_amd/autotune.py: lines of
+// "log2(N) index".  Read from $AXHEL_HIP_TUNING, else from
+// ntt_tuning_gfx950.txt in the package directory above this library.
+void load_tuning(const std::vector<std::vector<NttConfig>> &reg) {
+    std::string path;
+    if (const char *env = std::getenv("AXHEL_HIP_TUNING")) {
+        path = env;
+    } else {
+        Dl_info info;
+        if (dladdr(reinterpret_cast<const void *>(&load_tuning), &info) && info.dli_fname) {
+            std::string so = info.dli_fname;
+            const size_t a = so.find_last_of('/');
+            const std::string libdir = a == std::string::npos ? "." : so.substr(0, a);
+            const size_t b = libdir.find_last_of('/');
+            const std::string pkg = b == std::string::npos ? "." : libdir.substr(0, b);
+            path = pkg + "/ntt_tuning_gfx950.txt";
+        }
+    }
+    if (path.empty()) return;
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return;
+    char line[256];
+    while (std::fgets(line, sizeof(line), f)) {
+        if (line[0] == '#') continue;
+        int logn = -1, idx = -1;
+        if (std::sscanf(line, "%d %d", &logn, &idx) == 2 && logn >= 1 && logn <= (int)kMaxLogN && idx >= 0 &&
+            idx < (int)reg[logn].size())
+            g_selected[logn].store(idx);
+    }
+    std::fclose(f);
+}
+
 const std::vector<std::vector<NttConfig>> &registry() {
     std::call_once(g_reg_once, [] {
         g_reg = new std::vector<std::vector<NttConfig>>(build_registry());
         for (auto &s : g_selected) s.store(0);
+        load_tuning(*g_reg);
     });
     return *g_reg;
 }
@@ -200,7 +238,7 @@ NttJobDev make_job(const axhel_hip_plan *p, const uint64_t *src, uint64_t *dst, 
 }
 
 hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool normalize, const NttJobDev *jobs,
-                      const NttJobDev &inline_job, uint32_t count, hipStream_t stream, int only_pass) {
+                      const NttJobDev &inline_job, uint32_t count, hipStream_t stream, int only_pass, bool vec16) {
     if (count == 0) return hipSuccess;
     const uint64_t n = 1ull << logn;
     for (int p = 0; p < cfg.npass; ++p) {
@@ -218,6 +256,7 @@ hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool no
         a.first_pass = p == 0;
         a.normalize = (p == cfg.npass - 1) && normalize;
         a.final_inv = inverse && (p == cfg.npass - 1);
+        a.vec16 = vec16 ? 1u : 0u;
         uint64_t blocks;
         if (k.multi) blocks = (count + k.gpw - 1) / k.gpw;
         else blocks = (uint64_t)count * a.groups_per_limb / k.gpw;
@@ -281,7 +320,8 @@ axhel_hip_status run_one(const axhel_hip_plan *plan, uint64_t *result, const uin
     AXHEL_HIP_TRY(stg.input(operand, plan->n, &dop));
     AXHEL_HIP_TRY(stg.output(result, plan->n, &dres));
     const NttJobDev job = make_job(plan, dop, dres, inverse);
-    AXHEL_HIP_TRY(launch_ntt(config_for(plan->logn), plan->logn, inverse, normalize, nullptr, job, 1, st));
+    const bool vec16 = aligned16(dop) && aligned16(dres);
+    AXHEL_HIP_TRY(launch_ntt(config_for(plan->logn), plan->logn, inverse, normalize, nullptr, job, 1, st, -1, vec16));
     AXHEL_HIP_TRY(stg.finish());
     return AXHEL_HIP_OK;
 }
@@ -383,7 +423,7 @@ axhel_hip_status axhel_hip_ntt_config_select(uint64_t degree, int index) {
     if (!valid_degree(degree, &logn)) return AXHEL_HIP_INVALID_ARGUMENT;
     const auto &v = configs_for(logn);
     if (index < -1 || index >= (int)v.size()) return AXHEL_HIP_INVALID_ARGUMENT;
-    g_selected[logn].store(index < 0 ? 0 : index);
+    g_selected[logn].store(index < 0 ? 0 : index);  // -1: the first (built-in default) configuration
     return AXHEL_HIP_OK;
 }
 

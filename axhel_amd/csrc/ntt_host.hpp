@@ -61,9 +61,10 @@ axhel_dev::NttJobDev make_job(const axhel_hip_plan *p, const uint64_t *src, uint
 // Launches all passes of one transform family: `count` limbs of the same
 // degree and direction whose descriptors are `jobs` (device table) or, when
 // jobs == nullptr and count == 1, `inline_job`.  only_pass >= 0 launches
-// that pass alone (per-kernel timing).
+// that pass alone (per-kernel timing).  vec16: every src/dst is 16-byte
+// aligned (enables 16-byte global accesses).
 hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool normalize,
                       const axhel_dev::NttJobDev *jobs, const axhel_dev::NttJobDev &inline_job, uint32_t count,
-                      hipStream_t stream, int only_pass = -1);
+                      hipStream_t stream, int only_pass = -1, bool vec16 = false);
 
 }  // namespace axhel_host

@@ -63,6 +63,7 @@ struct PassArgs {
     uint32_t first_pass;       // read the job's src (else dst)
     uint32_t normalize;        // last pass: fold the [0,4q)->[0,q) / [0,2q)->[0,q) reduction in
     uint32_t final_inv;        // inverse pass holding bit logN-1 (scaled final stage)
+    uint32_t vec16;            // every job's src and dst are 16-byte aligned
 };
 
 __device__ __forceinline__ NttJobDev load_job(const PassArgs &a, uint32_t limb) {
@@ -137,6 +138,28 @@ struct Lds {
     }
 };
 
+// Global-memory accesses through address-space-1 pointers, so they issue as
+// global_* (not flat_*) instructions and do not count against LDS waits.
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) uint64_t gword;
+typedef __attribute__((address_space(1))) u64x2 gword2;
+
+__device__ __forceinline__ uint64_t ld_g(const uint64_t *p) { return *(const gword *)p; }
+__device__ __forceinline__ void st_g(uint64_t *p, uint64_t v) { *(gword *)p = v; }
+__device__ __forceinline__ ulonglong2 ld_g2(const void *p) {
+    const u64x2 x = *(const gword2 *)p;
+    ulonglong2 r;
+    r.x = x.x;
+    r.y = x.y;
+    return r;
+}
+__device__ __forceinline__ void st_g2(void *p, ulonglong2 v) {
+    u64x2 x;
+    x.x = v.x;
+    x.y = v.y;
+    *(gword2 *)p = x;
+}
+
 template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, bool MULTI>
 __global__ void __launch_bounds__(GPW << (LOG_G - LOG_E))
 ntt_pass_kernel(PassArgs a) {
@@ -181,8 +204,7 @@ ntt_pass_kernel(PassArgs a) {
     const bool valid = limb < a.num_limbs;
     const uint32_t jl = valid ? limb : 0;
     const NttJobDev J = load_job(a, jl);
-    const uint64_t q = J.q;
-    const uint64_t q2 = q << 1;
+    const ModConsts m = mod_consts(J.q);
     const uint64_t *src = a.first_pass ? J.src : J.dst;
     uint64_t *dst = J.dst;
     const ulonglong2 *tw = J.tw;
@@ -195,27 +217,40 @@ ntt_pass_kernel(PassArgs a) {
         if constexpr (COL) return (k << blo) | o;
         else return (o << LOG_G) | k;
     };
+    // ROW tiles of one limb are GPW * G contiguous words.
+    const uint32_t tile_o0 = MULTI ? 0u : tile * GPW - limb * a.groups_per_limb;
 
     uint64_t v[E];
 
     // ---- load: registers in chunk-0 layout
     constexpr bool LOAD_VIA_LDS = !COL && !FWD;  // inverse ROW: chunk 0 is thread-contiguous
     if constexpr (LOAD_VIA_LDS) {
-        // coalesced copy of the tile into LDS
-        for (uint32_t e = t; e < (uint32_t)(GPW * G); e += NT) {
-            const uint32_t ge = e >> LOG_G, ke = e & (G - 1);
-            const uint32_t gide = tile * GPW + ge;
-            uint32_t le, oe;
-            if constexpr (MULTI) {
-                le = gide / a.groups_per_limb;
-                oe = gide % a.groups_per_limb;
+        if constexpr (!MULTI) {
+            // contiguous tile -> LDS, 16-byte vectors when aligned
+            const uint64_t *ts = src + ((uint64_t)tile_o0 << LOG_G);
+            if (a.vec16) {
+#pragma unroll
+                for (int i = 0; i < E / 2; ++i) {
+                    const uint32_t e = 2u * (t + (uint32_t)i * NT);
+                    const ulonglong2 x2 = ld_g2(ts + e);
+                    const uint32_t ge = e >> LOG_G, ke = e & (G - 1);
+                    lds[L::idx(ge, ke)] = x2.x;
+                    lds[L::idx(ge, ke + 1)] = x2.y;
+                }
             } else {
-                le = limb;
-                oe = gide - limb * a.groups_per_limb;
+#pragma unroll
+                for (int i = 0; i < E; ++i) {
+                    const uint32_t e = t + (uint32_t)i * NT;
+                    lds[L::idx(e >> LOG_G, e & (G - 1))] = ld_g(ts + e);
+                }
             }
-            if (le < a.num_limbs) {
-                const uint64_t *se = MULTI ? job_src(a, le) : src;
-                lds[L::idx(ge, ke)] = se[(oe << LOG_G) | ke];
+        } else {
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                const uint32_t e = t + (uint32_t)i * NT;
+                const uint32_t ge = e >> LOG_G, ke = e & (G - 1);
+                const uint32_t le = tile * GPW + ge;  // one group per limb
+                if (le < a.num_limbs) lds[L::idx(ge, ke)] = ld_g(job_src(a, le) + ke);
             }
         }
         __syncthreads();
@@ -229,7 +264,7 @@ ntt_pass_kernel(PassArgs a) {
         if (valid) {
             static_for<0, E>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
-                v[r] = src[gaddr(k0 + Gm::kr(0, r))];
+                v[r] = ld_g(src + gaddr(k0 + Gm::kr(0, r)));
             });
         } else {
             static_for<0, E>([&](auto rc) { v[decltype(rc)::value] = 0; });
@@ -269,6 +304,7 @@ ntt_pass_kernel(PassArgs a) {
             if constexpr (COL) pt = ktid >> (p + 1);
             else pt = ((o << LOG_G) | ktid) >> (p + 1);
             const bool fin = !FWD && a.final_inv && (p == LOG_G - 1);
+            const uint32_t tbase = FWD ? (n >> (gb + 1)) + pt : n + 1u - (n >> gb) + pt;
             uint64_t w[E], wq[E];
             static_for<0, E>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
@@ -277,21 +313,18 @@ ntt_pass_kernel(PassArgs a) {
                     constexpr uint32_t pr = Gm::kr(c, r) >> (p + 1);
                     if constexpr (rp == r) {
                         if (!fin) {
-                            uint32_t ti;
-                            if constexpr (FWD) ti = (n >> (gb + 1)) + pt + pr;
-                            else ti = n + 1u - (n >> gb) + pt + pr;
-                            const ulonglong2 tv = tw[ti];
+                            const ulonglong2 tv = ld_g2(tw + tbase + pr);
                             w[r] = tv.x;
                             wq[r] = tv.y;
                         }
                     }
                     if constexpr (FWD) {
-                        fwd_bfly(v[r], v[r | bit], w[rp], wq[rp], q, q2);
+                        fwd_bfly(v[r], v[r | bit], w[rp], wq[rp], m);
                     } else {
                         if (fin)
-                            inv_final_bfly(v[r], v[r | bit], J.ninv, J.ninv_q, J.sroot, J.sroot_q, q, q2);
+                            inv_final_bfly(v[r], v[r | bit], J.ninv, J.ninv_q, J.sroot, J.sroot_q, m);
                         else
-                            inv_bfly(v[r], v[r | bit], w[rp], wq[rp], q, q2);
+                            inv_bfly(v[r], v[r | bit], w[rp], wq[rp], m);
                     }
                 }
             });
@@ -302,8 +335,8 @@ ntt_pass_kernel(PassArgs a) {
     if (a.normalize) {
         static_for<0, E>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
-            if constexpr (FWD) v[r] = csub(csub(v[r], q2), q);
-            else v[r] = csub(v[r], q);
+            if constexpr (FWD) v[r] = csub_n(csub_n(v[r], m.q2, m.nq2), m.q, m.nq);
+            else v[r] = csub_n(v[r], m.q, m.nq);
         });
     }
 
@@ -317,20 +350,32 @@ ntt_pass_kernel(PassArgs a) {
             lds[L::idx(g, kl + Gm::kr(NC - 1, r))] = v[r];
         });
         __syncthreads();
-        for (uint32_t e = t; e < (uint32_t)(GPW * G); e += NT) {
-            const uint32_t ge = e >> LOG_G, ke = e & (G - 1);
-            const uint32_t gide = tile * GPW + ge;
-            uint32_t le, oe;
-            if constexpr (MULTI) {
-                le = gide / a.groups_per_limb;
-                oe = gide % a.groups_per_limb;
+        if constexpr (!MULTI) {
+            uint64_t *td = dst + ((uint64_t)tile_o0 << LOG_G);
+            if (a.vec16) {
+#pragma unroll
+                for (int i = 0; i < E / 2; ++i) {
+                    const uint32_t e = 2u * (t + (uint32_t)i * NT);
+                    const uint32_t ge = e >> LOG_G, ke = e & (G - 1);
+                    ulonglong2 x2;
+                    x2.x = lds[L::idx(ge, ke)];
+                    x2.y = lds[L::idx(ge, ke + 1)];
+                    st_g2(td + e, x2);
+                }
             } else {
-                le = limb;
-                oe = gide - limb * a.groups_per_limb;
+#pragma unroll
+                for (int i = 0; i < E; ++i) {
+                    const uint32_t e = t + (uint32_t)i * NT;
+                    st_g(td + e, lds[L::idx(e >> LOG_G, e & (G - 1))]);
+                }
             }
-            if (le < a.num_limbs) {
-                uint64_t *de = MULTI ? job_dst(a, le) : dst;
-                de[(oe << LOG_G) | ke] = lds[L::idx(ge, ke)];
+        } else {
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                const uint32_t e = t + (uint32_t)i * NT;
+                const uint32_t ge = e >> LOG_G, ke = e & (G - 1);
+                const uint32_t le = tile * GPW + ge;
+                if (le < a.num_limbs) st_g(job_dst(a, le) + ke, lds[L::idx(ge, ke)]);
             }
         }
     } else {
@@ -338,7 +383,7 @@ ntt_pass_kernel(PassArgs a) {
             const uint32_t kl = Gm::kt(NC - 1, tid);
             static_for<0, E>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
-                dst[gaddr(kl + Gm::kr(NC - 1, r))] = v[r];
+                st_g(dst + gaddr(kl + Gm::kr(NC - 1, r)), v[r]);
             });
         }
     }

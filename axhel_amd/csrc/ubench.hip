@@ -14,7 +14,7 @@ namespace {
 constexpr int kChains = 8;  // independent butterflies in flight per lane
 
 __global__ void __launch_bounds__(256) bfly_peak_kernel(uint64_t *out, uint64_t q, uint32_t iters) {
-    const uint64_t q2 = q << 1;
+    const axhel_dev::ModConsts m = axhel_dev::mod_consts(q);
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t x[kChains], y[kChains];
     // per-lane twiddle and Shoup quotient (valid pair: w < q, wq = floor(w 2^64 / q)
@@ -28,7 +28,7 @@ __global__ void __launch_bounds__(256) bfly_peak_kernel(uint64_t *out, uint64_t 
     }
     for (uint32_t it = 0; it < iters; ++it) {
 #pragma unroll
-        for (int c = 0; c < kChains; ++c) axhel_dev::fwd_bfly(x[c], y[c], w, wq, q, q2);
+        for (int c = 0; c < kChains; ++c) axhel_dev::fwd_bfly(x[c], y[c], w, wq, m);
     }
     uint64_t acc = 0;
 #pragma unroll
