@@ -62,6 +62,33 @@ class NttConfig(ctypes.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class ServeConfig(ctypes.Structure):
+    """axhel_hip_serve_config (include/axhel_hip.h)."""
+    _fields_ = [("memory_budget_bytes", _u64), ("max_batch_limbs", ctypes.c_uint32), ("slots", ctypes.c_uint32),
+                ("backend", ctypes.c_uint32), ("null_latency_us", ctypes.c_uint32), ("fail_every_alloc", _u64)]
+
+
+class Request(ctypes.Structure):
+    """axhel_hip_request (include/axhel_hip.h)."""
+    _fields_ = [("op", ctypes.c_int32), ("limbs", ctypes.c_uint32), ("degree", _u64), ("plans", _p),
+                ("moduli", _p), ("scalars", _p), ("result", _p), ("operand1", _p), ("operand2", _p),
+                ("input_mod_factor", _u64), ("output_mod_factor", _u64), ("timeout_us", ctypes.c_int64)]
+
+
+class ServeStats(ctypes.Structure):
+    """axhel_hip_serve_stats (include/axhel_hip.h)."""
+    _fields_ = [(k, _u64) for k in ("submitted", "completed", "rejected", "timed_out", "cancelled", "failed",
+                                    "limbs_completed", "batches", "bytes_in_use", "peak_bytes_in_use")] + \
+               [(k, ctypes.c_double) for k in ("p50_us", "p99_us", "mean_us", "max_us",
+                                              "first_arrival_to_last_completion_s")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+OP_NTT_FORWARD, OP_NTT_INVERSE, OP_ADD, OP_SUB, OP_MUL, OP_FMA, OP_REDUCE = range(7)
+BACKEND_DEVICE, BACKEND_NULL = 0, 1
+
 # name -> (restype, argtypes); must cover every function declared in include/axhel_hip.h
 SIGNATURES = {
     "axhel_hip_status_string": (ctypes.c_char_p, [_st]),
@@ -95,6 +122,14 @@ SIGNATURES = {
     "axhel_hip_ntt_config_get": (_st, [_u64, ctypes.c_int, ctypes.POINTER(NttConfig)]),
     "axhel_hip_ntt_config_select": (_st, [_u64, ctypes.c_int]),
     "axhel_hip_ntt_config_selected": (ctypes.c_int, [_u64]),
+    "axhel_hip_server_create": (_st, [ctypes.POINTER(ServeConfig), ctypes.POINTER(_p)]),
+    "axhel_hip_server_destroy": (_st, [_p]),
+    "axhel_hip_server_plan": (_st, [_p, _u64, _u64, _u64, ctypes.POINTER(_p)]),
+    "axhel_hip_server_submit": (_st, [_p, ctypes.POINTER(Request), ctypes.POINTER(_u64)]),
+    "axhel_hip_server_wait": (_st, [_p, _u64, ctypes.c_int64]),
+    "axhel_hip_server_cancel": (_st, [_p, _u64]),
+    "axhel_hip_server_stats": (_st, [_p, ctypes.POINTER(ServeStats)]),
+    "axhel_hip_server_reset_stats": (_st, [_p]),
     "axhel_hip_ubench_butterfly_peak": (_st, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
 }
 
@@ -320,6 +355,80 @@ def butterfly_peak():
     r, s = ctypes.c_double(), ctypes.c_double()
     _check(_load().axhel_hip_ubench_butterfly_peak(ctypes.byref(r), ctypes.byref(s)), "butterfly_peak")
     return r.value, s.value
+
+
+# ------------------------------------------------------------------ serving
+class Server:
+    """Continuous-batching request loop (axhel_hip_server_*).
+
+    submit() takes one request over `limbs` vectors of `degree` words (limb l
+    at offset l*degree of every buffer) and returns a ticket; wait(ticket)
+    returns the final status code (OK, TIMEOUT, CANCELLED, NO_MEMORY, ...)."""
+
+    def __init__(self, memory_budget_bytes=0, max_batch_limbs=0, slots=0, backend=BACKEND_DEVICE,
+                 null_latency_us=0, fail_every_alloc=0):
+        cfg = ServeConfig(memory_budget_bytes, max_batch_limbs, slots, backend, null_latency_us, fail_every_alloc)
+        h = _p()
+        _check(_load().axhel_hip_server_create(ctypes.byref(cfg), ctypes.byref(h)), "server_create")
+        self._h = h
+
+    def plan(self, degree, modulus, root):
+        p = _p()
+        _check(_load().axhel_hip_server_plan(self._h, degree, modulus, root, ctypes.byref(p)), "server_plan")
+        return p.value
+
+    def submit(self, op, degree, limbs, result, operand1, operand2=None, plans=None, moduli=None, scalars=None,
+               input_mod_factor=1, output_mod_factor=1, timeout_us=0):
+        """Returns (status, ticket); ticket is None unless status == OK."""
+        keep = []
+
+        def arr(vals):
+            if vals is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(vals, dtype=np.uint64))
+            keep.append(a)
+            return a.ctypes.data
+
+        r = Request()
+        r.op, r.limbs, r.degree = op, limbs, degree
+        if plans is not None:
+            pa = (_p * limbs)(*[p if isinstance(p, int) else p.handle for p in plans])
+            keep.append(pa)
+            r.plans = ctypes.cast(pa, _p)
+        r.moduli = arr(moduli)
+        r.scalars = arr(scalars)
+        r.result = result if isinstance(result, int) else _ptr(result)
+        r.operand1 = operand1 if isinstance(operand1, int) else _ptr(operand1)
+        r.operand2 = operand2 if (operand2 is None or isinstance(operand2, int)) else _ptr(operand2)
+        r.input_mod_factor, r.output_mod_factor, r.timeout_us = input_mod_factor, output_mod_factor, timeout_us
+        t = _u64()
+        s = _load().axhel_hip_server_submit(self._h, ctypes.byref(r), ctypes.byref(t))
+        return s, (t.value if s == OK else None)
+
+    def wait(self, ticket, timeout_us=-1):
+        return int(_load().axhel_hip_server_wait(self._h, ticket, timeout_us))
+
+    def cancel(self, ticket):
+        return int(_load().axhel_hip_server_cancel(self._h, ticket))
+
+    def stats(self):
+        st = ServeStats()
+        _check(_load().axhel_hip_server_stats(self._h, ctypes.byref(st)), "server_stats")
+        return st.as_dict()
+
+    def reset_stats(self):
+        _check(_load().axhel_hip_server_reset_stats(self._h), "server_reset_stats")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _load().axhel_hip_server_destroy(self._h)
+            self._h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------------ tuning

@@ -3,6 +3,7 @@
   axhel_amd/lib/libaxhel_hip.so  HIP kernels + the C ABI (include/axhel_hip.h)
   axhel_amd/lib/libaxhel.so      the unipi::axhel C++ drop-in API over the C ABI
   axhel_amd/lib/axhel_cxx_check  a C++ program exercising the drop-in API
+  axhel_amd/lib/axhel_serve_bench  the serving-loop load driver (Poisson arrivals)
 
 Usage: python -m axhel_amd.build [-v] [-j N]
 """
@@ -69,6 +70,13 @@ def build(verbose=False, jobs=None):
           os.path.join(CSRC, "cxx_check.cpp"), "-L", LIBDIR, "-laxhel", "-laxhel_hip",
           "-Wl,-rpath,$ORIGIN", "-o", check + ".tmp"], verbose)
     os.replace(check + ".tmp", check)
+
+    # serving benchmark driver (host program over the C ABI)
+    sb = os.path.join(LIBDIR, "axhel_serve_bench")
+    _run([HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-Wno-unused-command-line-argument",
+          "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "serve_bench.hip"),
+          "-L", LIBDIR, "-laxhel_hip", "-lpthread", "-Wl,-rpath,$ORIGIN", "-o", sb + ".tmp"], verbose)
+    os.replace(sb + ".tmp", sb)
     return hip_lib
 
 

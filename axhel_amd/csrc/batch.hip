@@ -62,14 +62,16 @@ hipError_t issue(axhel_hip_batch *b, hipStream_t stream) {
     return hipSuccess;
 }
 
-// Capture: fork each group onto a side stream so independent groups become
-// parallel graph branches, then join back on the origin stream.
+// Capture.  One group: its passes are captured straight on the origin stream
+// (a plain chain of kernel nodes).  Several groups: each is forked onto a
+// side stream so independent groups become parallel graph branches, joined
+// back on the origin stream.
 hipError_t capture(axhel_hip_batch *b) {
     static const NttJobDev none{};
     hipStream_t origin = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&origin, hipStreamNonBlocking);
     if (e != hipSuccess) return e;
-    const int nb = std::min<int>(kBranches, (int)b->groups.size());
+    const int nb = b->groups.size() > 1 ? std::min<int>(kBranches, (int)b->groups.size()) : 0;
     for (int i = 0; i < nb; ++i) {
         hipStream_t s;
         hipEvent_t ev;
@@ -79,19 +81,26 @@ hipError_t capture(axhel_hip_batch *b) {
         b->events.push_back(ev);
     }
     hipEvent_t fork_ev = nullptr;
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming);
+    if (e == hipSuccess && nb) e = hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        hipError_t ce = hipEventRecord(fork_ev, origin);
-        for (int i = 0; i < nb && ce == hipSuccess; ++i) ce = hipStreamWaitEvent(b->side[i], fork_ev, 0);
-        for (size_t gi = 0; gi < b->groups.size() && ce == hipSuccess; ++gi) {
-            const auto &g = b->groups[gi];
+        hipError_t ce = hipSuccess;
+        if (nb == 0) {
+            const auto &g = b->groups[0];
             ce = launch_ntt(config_for(g.logn), g.logn, g.inverse, g.normalize, b->d_jobs + g.offset, none, g.count,
-                            b->side[gi % nb], -1, g.vec16);
-        }
-        for (int i = 0; i < nb && ce == hipSuccess; ++i) {
-            ce = hipEventRecord(b->events[i], b->side[i]);
-            if (ce == hipSuccess) ce = hipStreamWaitEvent(origin, b->events[i], 0);
+                            origin, -1, g.vec16);
+        } else {
+            ce = hipEventRecord(fork_ev, origin);
+            for (int i = 0; i < nb && ce == hipSuccess; ++i) ce = hipStreamWaitEvent(b->side[i], fork_ev, 0);
+            for (size_t gi = 0; gi < b->groups.size() && ce == hipSuccess; ++gi) {
+                const auto &g = b->groups[gi];
+                ce = launch_ntt(config_for(g.logn), g.logn, g.inverse, g.normalize, b->d_jobs + g.offset, none,
+                                g.count, b->side[gi % nb], -1, g.vec16);
+            }
+            for (int i = 0; i < nb && ce == hipSuccess; ++i) {
+                ce = hipEventRecord(b->events[i], b->side[i]);
+                if (ce == hipSuccess) ce = hipStreamWaitEvent(origin, b->events[i], 0);
+            }
         }
         e = hipStreamEndCapture(origin, &b->graph);
         if (ce != hipSuccess) e = ce;

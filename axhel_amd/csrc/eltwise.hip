@@ -11,6 +11,7 @@
 
 #include "arith.hpp"
 #include "common.hpp"
+#include "eltwise_host.hpp"
 
 using axhel_dev::Barrett;
 
@@ -194,6 +195,89 @@ axhel_hip_status copy_words(uint64_t *r, const uint64_t *a, uint64_t n, void *st
 }
 
 }  // namespace
+
+// ------------------------------------------------------ batched element-wise
+
+// Several limbs of one element-wise op in one launch: limb j covers n words
+// with its own modulus; grid = limbs x chunks.  Used by the serving loop for
+// RNS requests (one request = `limbs` vectors, one modulus each).
+namespace {
+
+static_assert((int)OP_RED_BARRETT == (int)axhel_host::ELT_RED_BARRETT, "op codes shared with eltwise_host.hpp");
+
+constexpr int kBatchChunk = kBlock * kUnroll;  // words per workgroup
+
+template <int OP, int F>
+__global__ void __launch_bounds__(kBlock)
+elt_batch_kernel(const axhel_host::EltLimbDev *limbs, uint64_t n, uint32_t chunks) {
+    const uint32_t j = blockIdx.x / chunks;
+    const uint64_t c0 = (uint64_t)(blockIdx.x % chunks) * kBatchChunk;
+    const axhel_host::EltLimbDev L = limbs[j];
+    uint64_t xa[kUnroll], xb[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t i = c0 + (uint64_t)u * kBlock + threadIdx.x;
+        if (i < n) {
+            xa[u] = L.a[i];
+            if constexpr (has_b(OP)) xb[u] = L.b[i];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t i = c0 + (uint64_t)u * kBlock + threadIdx.x;
+        if (i < n) L.r[i] = elt_apply<OP, F>(xa[u], has_b(OP) ? xb[u] : 0, L.s, L.bc);
+    }
+}
+
+template <int OP, int F>
+hipError_t launch_batch(const axhel_host::EltLimbDev *d_limbs, uint32_t count, uint64_t n, hipStream_t st) {
+    const uint64_t chunks = (n + kBatchChunk - 1) / kBatchChunk;
+    const uint64_t blocks = chunks * count;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((elt_batch_kernel<OP, F>), dim3((unsigned)blocks), dim3(kBlock), 0, st, d_limbs, n,
+                       (uint32_t)chunks);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+namespace axhel_host {
+
+Barrett barrett_for(uint64_t q) { return make_barrett(q); }
+
+// op: 0 add_vv, 1 add_vs, 2 sub_vv, 3 sub_vs, 4 mul, 5 fma, 6 fma (null addend),
+// 7.. reduce modes (7: 2->1, 8: 4->1, 9: 4->2, 10: Barrett).  f = input factor.
+hipError_t launch_elt_batch(int op, int f, const EltLimbDev *d_limbs, uint32_t count, uint64_t n, hipStream_t st) {
+#define AXHEL_EB(OPV, FV) return launch_batch<OPV, FV>(d_limbs, count, n, st)
+    switch (op) {
+        case OP_ADD_VV: AXHEL_EB(OP_ADD_VV, 1);
+        case OP_ADD_VS: AXHEL_EB(OP_ADD_VS, 1);
+        case OP_SUB_VV: AXHEL_EB(OP_SUB_VV, 1);
+        case OP_SUB_VS: AXHEL_EB(OP_SUB_VS, 1);
+        case OP_MUL:
+            if (f == 1) AXHEL_EB(OP_MUL, 1);
+            if (f == 2) AXHEL_EB(OP_MUL, 2);
+            AXHEL_EB(OP_MUL, 4);
+        case OP_FMA:
+            if (f == 1) AXHEL_EB(OP_FMA, 1);
+            if (f == 2) AXHEL_EB(OP_FMA, 2);
+            if (f == 4) AXHEL_EB(OP_FMA, 4);
+            AXHEL_EB(OP_FMA, 8);
+        case OP_FMA_NULL:
+            if (f == 1) AXHEL_EB(OP_FMA_NULL, 1);
+            if (f == 2) AXHEL_EB(OP_FMA_NULL, 2);
+            if (f == 4) AXHEL_EB(OP_FMA_NULL, 4);
+            AXHEL_EB(OP_FMA_NULL, 8);
+        case OP_RED_2_1: AXHEL_EB(OP_RED_2_1, 1);
+        case OP_RED_4_1: AXHEL_EB(OP_RED_4_1, 1);
+        case OP_RED_4_2: AXHEL_EB(OP_RED_4_2, 1);
+        case OP_RED_BARRETT: AXHEL_EB(OP_RED_BARRETT, 1);
+    }
+#undef AXHEL_EB
+    return hipErrorInvalidValue;
+}
+
+}  // namespace axhel_host
 
 extern "C" {
 

@@ -95,11 +95,24 @@ def eltwise_leg(ax, torch, src, dst, q, reps):
     return out
 
 
+def serve_leg(ax, seconds):
+    """Serving loop under Poisson load (native driver axhel_serve_bench):
+    capacity from a closed loop, then p50/p99 at 0.5x and 0.8x of it."""
+    import subprocess
+    exe = os.path.join(ax.LIB_DIR, "axhel_serve_bench")
+    r = subprocess.run([exe, "--seconds", str(seconds), "--fractions", "0.5,0.8"], capture_output=True, text=True,
+                       timeout=120 + 4 * seconds)
+    if r.returncode != 0:
+        log("serve bench failed:", r.stderr[-2000:])
+        return {"error": r.returncode}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--polys", type=int, default=64)
     ap.add_argument("--limbs", type=int, default=24)
     ap.add_argument("--logn", type=int, default=16)
@@ -107,6 +120,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eltwise", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run one 2^27-word EltwiseReduceMod(4->2) (1 GiB in, 1 GiB out): "
                          "the known-size stream that scales rocprof FETCH_SIZE/WRITE_SIZE")
@@ -147,25 +162,41 @@ def main():
     batch = ax.Batch(jobs)
     limbs = P * L
 
+    # the timed steps run on a dedicated stream (graph replays on the NULL
+    # stream pick up its implicit synchronisation)
+    run_stream = torch.cuda.Stream()
     for _ in range(args.warmup):
-        batch.launch()
+        batch.launch(run_stream)
+        batch.launch_eager(run_stream)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        batch.launch()
+        batch.launch(run_stream)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms = el * 1e3 / args.steps
     value = limbs * args.steps / el
-    log(f"batched forward: {ms:.3f} ms/step, {value:.4g} limbs/s")
+    log(f"batched forward (graph replay): {ms:.3f} ms/step, {value:.4g} limbs/s")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.launch_eager(run_stream)
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log(f"same step as eager launches: {eager_ms:.3f} ms/step")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.launch()
+    torch.cuda.synchronize()
+    null_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log(f"graph replay on the NULL stream: {null_ms:.3f} ms/step")
 
     # inverse (reported beside)
     ibatch = ax.Batch([(plans[l], src[p, l], dst[p, l], True, 1) for p in range(P) for l in range(L)])
-    ibatch.launch()
+    ibatch.launch(run_stream)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ibatch.launch()
+        ibatch.launch(run_stream)
     torch.cuda.synchronize()
     inv_rate = limbs * args.steps / (time.perf_counter() - t0)
     # restore the forward inputs (inverse of forward is the identity)
@@ -227,6 +258,11 @@ def main():
         elt = eltwise_leg(ax, torch, src, dst, primes[0], reps=max(3, min(10, args.steps)))
         log("eltwise GB/s:", {k: round(v["GB/s"], 1) for k, v in elt.items()})
 
+    serving = None
+    if not args.no_serve:
+        serving = serve_leg(ax, args.serve_seconds)
+        log("serving:", json.dumps(serving)[:400])
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, primes, psis, args.cpu_seconds)
@@ -252,6 +288,8 @@ def main():
         "batched_vs_plain": value / plain,
         "inverse_limbs_per_s": inv_rate,
         "kernel_ms": kdur,
+        "eager_ms_per_step": eager_ms,
+        "null_stream_graph_ms_per_step": null_ms,
         "roofline": {"bound": "valu", "achieved": achieved_bfly, "peak": peak_bfly, "unit": "Gbfly/s",
                      "frac": achieved_bfly / peak_bfly, "traffic": traffic,
                      "kernel": f"ntt pass {dom} of {kcount} ({kdur[dom]:.4f} ms)",
@@ -259,6 +297,7 @@ def main():
                      "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved_gbs / HBM_PEAK_GBS}},
         "eltwise": elt,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -154,6 +154,96 @@ axhel_hip_status axhel_hip_ntt_batch_launch_kernel(axhel_hip_batch *batch, int i
 /* One-shot batch: create, launch eagerly on `stream`, synchronize, destroy. */
 axhel_hip_status axhel_hip_ntt_batch_run(const axhel_hip_ntt_job *jobs, size_t count, void *stream);
 
+/* ---------------------------------------------------------------- serving */
+
+/* Request loop with continuous batching.  No reference counterpart: the
+   reference is a synchronous single-call library; this is the caller loop
+   (integration/seal/SEAL-4.4.5-AXHEL.patch:185-226 runs it per limb, one call
+   at a time) moved onto the device.  A scheduler thread admits requests in
+   arrival order while memory allows, packs every admitted request into the
+   next device batch (one kernel launch per (op, degree) group), keeps up to
+   `slots` batches in flight on separate streams, and completes requests as
+   their batch finishes, so arrivals join while earlier work runs. */
+
+typedef enum axhel_hip_op {
+    AXHEL_HIP_OP_NTT_FORWARD = 0, /* ComputeForward per limb (plans[l]) */
+    AXHEL_HIP_OP_NTT_INVERSE = 1, /* ComputeInverse per limb */
+    AXHEL_HIP_OP_ADD = 2,         /* EltwiseAddMod vv per limb (moduli[l]) */
+    AXHEL_HIP_OP_SUB = 3,         /* EltwiseSubMod vv */
+    AXHEL_HIP_OP_MUL = 4,         /* EltwiseMulMod, input_mod_factor */
+    AXHEL_HIP_OP_FMA = 5,         /* EltwiseFMAMod, scalars[l], operand2 = addend or NULL */
+    AXHEL_HIP_OP_REDUCE = 6       /* EltwiseReduceMod arithmetic branches (2->1, 4->1, 4->2); input_mod_factor
+                                     0 = each limb's Barrett branch (in_mod_factor == modulus) */
+} axhel_hip_op;
+
+typedef enum axhel_hip_backend {
+    AXHEL_HIP_BACKEND_DEVICE = 0,
+    /* Scheduler only: no GPU is touched and no result is computed; batches
+       "finish" after null_latency_us.  For testing admission, timeouts,
+       cancellation and fault injection on machines without a GPU. */
+    AXHEL_HIP_BACKEND_NULL = 1
+} axhel_hip_backend;
+
+typedef struct axhel_hip_serve_config {
+    uint64_t memory_budget_bytes; /* device bytes for staging host operands; 0 = half the free HBM */
+    uint32_t max_batch_limbs;     /* limbs packed into one device batch; 0 = 8192 */
+    uint32_t slots;               /* device batches in flight; 0 = 3 */
+    uint32_t backend;             /* axhel_hip_backend */
+    uint32_t null_latency_us;     /* NULL backend: simulated batch duration */
+    uint64_t fail_every_alloc;    /* fault injection: every k-th staging allocation fails; 0 = never */
+} axhel_hip_serve_config;
+
+/* One request: `limbs` vectors of `degree` words, limb l at offset l*degree
+   in every buffer.  Buffers may be host or device memory; host buffers are
+   staged through the server's device pool (and count against the budget).
+   The arrays must stay valid until the request completes. */
+typedef struct axhel_hip_request {
+    int32_t op;    /* axhel_hip_op */
+    uint32_t limbs;
+    uint64_t degree;
+    const axhel_hip_plan *const *plans; /* NTT ops: one plan per limb, of this degree */
+    const uint64_t *moduli;             /* element-wise ops: one modulus per limb */
+    const uint64_t *scalars;            /* FMA: one scalar per limb */
+    uint64_t *result;
+    const uint64_t *operand1;
+    const uint64_t *operand2; /* ADD/SUB/MUL: second vector; FMA: addend (NULL allowed) */
+    uint64_t input_mod_factor;
+    uint64_t output_mod_factor;
+    int64_t timeout_us; /* dropped with AXHEL_HIP_TIMEOUT if not started in time; <= 0: no limit */
+} axhel_hip_request;
+
+typedef struct axhel_hip_serve_stats {
+    uint64_t submitted, completed, rejected, timed_out, cancelled, failed;
+    uint64_t limbs_completed, batches;
+    uint64_t bytes_in_use, peak_bytes_in_use;
+    double p50_us, p99_us, mean_us, max_us; /* arrival -> completion, completed requests */
+    double first_arrival_to_last_completion_s;
+} axhel_hip_serve_stats;
+
+typedef struct axhel_hip_server axhel_hip_server;
+
+axhel_hip_status axhel_hip_server_create(const axhel_hip_serve_config *config, axhel_hip_server **server);
+/* Stops accepting requests, finishes every admitted one, fails the queued
+   ones with AXHEL_HIP_CANCELLED, and frees the server. */
+axhel_hip_status axhel_hip_server_destroy(axhel_hip_server *server);
+/* Resident plan cache keyed by (degree, modulus, root), owned by the server. */
+axhel_hip_status axhel_hip_server_plan(axhel_hip_server *server, uint64_t degree, uint64_t modulus,
+                                       uint64_t root_of_unity, const axhel_hip_plan **plan);
+/* Queues a request; *ticket identifies it.  Rejected at once (no ticket)
+   when invalid or larger than the whole memory budget. */
+axhel_hip_status axhel_hip_server_submit(axhel_hip_server *server, const axhel_hip_request *request,
+                                         uint64_t *ticket);
+/* Waits up to timeout_us (< 0: forever) for the request; on completion
+   returns its final status and forgets the ticket.  AXHEL_HIP_TIMEOUT here
+   means "still running" (the ticket stays valid). */
+axhel_hip_status axhel_hip_server_wait(axhel_hip_server *server, uint64_t ticket, int64_t timeout_us);
+/* Cancels a queued request (AXHEL_HIP_OK), or reports AXHEL_HIP_REJECTED if
+   it already runs or finished. */
+axhel_hip_status axhel_hip_server_cancel(axhel_hip_server *server, uint64_t ticket);
+axhel_hip_status axhel_hip_server_stats(axhel_hip_server *server, axhel_hip_serve_stats *stats);
+/* Clears latency samples and counters (after a warm-up). */
+axhel_hip_status axhel_hip_server_reset_stats(axhel_hip_server *server);
+
 /* ---------------------------------------------------------------- tuning */
 
 /* Kernel configuration for one degree.  A transform runs as one ROW pass
