@@ -62,6 +62,12 @@ class NttConfig(ctypes.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class PolymulJob(ctypes.Structure):
+    """axhel_hip_polymul_job (include/axhel_hip.h)."""
+    _fields_ = [("plan", _p), ("result", _p), ("a", _p), ("b", _p), ("addend", _p), ("scalar", _u64),
+                ("scratch", _p)]
+
+
 class ServeConfig(ctypes.Structure):
     """axhel_hip_serve_config (include/axhel_hip.h)."""
     _fields_ = [("memory_budget_bytes", _u64), ("max_batch_limbs", ctypes.c_uint32), ("slots", ctypes.c_uint32),
@@ -122,6 +128,10 @@ SIGNATURES = {
     "axhel_hip_ntt_config_get": (_st, [_u64, ctypes.c_int, ctypes.POINTER(NttConfig)]),
     "axhel_hip_ntt_config_select": (_st, [_u64, ctypes.c_int]),
     "axhel_hip_ntt_config_selected": (ctypes.c_int, [_u64]),
+    "axhel_hip_polymul_create": (_st, [ctypes.POINTER(PolymulJob), ctypes.c_size_t, ctypes.c_int,
+                                       ctypes.POINTER(_p)]),
+    "axhel_hip_polymul_launch": (_st, [_p, _p]),
+    "axhel_hip_polymul_destroy": (_st, [_p]),
     "axhel_hip_server_create": (_st, [ctypes.POINTER(ServeConfig), ctypes.POINTER(_p)]),
     "axhel_hip_server_destroy": (_st, [_p]),
     "axhel_hip_server_plan": (_st, [_p, _u64, _u64, _u64, ctypes.POINTER(_p)]),
@@ -355,6 +365,44 @@ def butterfly_peak():
     r, s = ctypes.c_double(), ctypes.c_double()
     _check(_load().axhel_hip_ubench_butterfly_peak(ctypes.byref(r), ctypes.byref(s)), "butterfly_peak")
     return r.value, s.value
+
+
+# ------------------------------------------------------- polynomial product
+class Polymul:
+    """Batched RNS polynomial product: per job (ntt, result, a, b, addend or
+    None, scalar, scratch) computes InverseNTT(FMA(Mul(NTT(a), NTT(b)), scalar,
+    addend)) with device tensors of ntt.degree words."""
+
+    def __init__(self, jobs, fused=True):
+        jobs = list(jobs)
+        arr = (PolymulJob * max(1, len(jobs)))()
+        self._keep = []
+        for i, (ntt, res, a, b, c, s, scr) in enumerate(jobs):
+            arr[i].plan = ntt.handle
+            arr[i].result = _ptr(res)
+            arr[i].a = _ptr(a)
+            arr[i].b = _ptr(b)
+            arr[i].addend = None if c is None else _ptr(c)
+            arr[i].scalar = s
+            arr[i].scratch = _ptr(scr)
+            self._keep.append(ntt)
+        h = _p()
+        _check(_load().axhel_hip_polymul_create(arr, len(jobs), 1 if fused else 0, ctypes.byref(h)), "polymul_create")
+        self._h = h
+
+    def launch(self, stream=None):
+        _check(_load().axhel_hip_polymul_launch(self._h, _stream_ptr(stream)), "polymul_launch")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _load().axhel_hip_polymul_destroy(self._h)
+            self._h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------------ serving

@@ -13,6 +13,7 @@
 #include <string>
 
 #include "ntt_host.hpp"
+#include "polymul_kernels.hpp"
 
 using axhel_dev::NttJobDev;
 using axhel_dev::PassArgs;
@@ -87,6 +88,20 @@ PassKernel pk() {
 }
 
 template <int LG, int LE, int GPW>
+PassKernel pk_fused() {
+    PassKernel k;
+    k.fn = reinterpret_cast<const void *>(&axhel_dev::polymul_row_kernel<LG, LE, GPW>);
+    k.log_g = LG;
+    k.log_e = LE;
+    k.gpw = GPW;
+    k.col = false;
+    k.multi = false;
+    k.threads = GPW << (LG - LE);
+    k.lds_bytes = (size_t)axhel_dev::Lds<LG, LE, GPW, false>::WORDS * sizeof(uint64_t);
+    return k;
+}
+
+template <int LG, int LE, int GPW>
 NttConfig single() {
     NttConfig c;
     c.pub.log_elems = LE;
@@ -96,6 +111,7 @@ NttConfig single() {
     c.npass = 1;
     c.fwd[0] = pk<LG, LE, GPW, false, (GPW > 1), true>();
     c.inv[0] = pk<LG, LE, GPW, false, (GPW > 1), false>();
+    if constexpr (GPW == 1 && LG >= 12) c.fused = pk_fused<LG, LE, GPW>();
     return c;
 }
 
@@ -112,6 +128,7 @@ NttConfig two() {
     c.fwd[1] = pk<LR, LER, GR, false, false, true>();
     c.inv[0] = pk<LR, LER, GR, false, false, false>();
     c.inv[1] = pk<LC, LEC, GC, true, false, false>();
+    c.fused = pk_fused<LR, LER, GR>();
     return c;
 }
 
@@ -221,6 +238,7 @@ hipError_t prepare_config(const NttConfig &cfg) {
         if (e == hipSuccess) e = ensure_lds(cfg.inv[p]);
         if (e != hipSuccess) return e;
     }
+    if (cfg.fused.fn) return ensure_lds(cfg.fused);
     return hipSuccess;
 }
 

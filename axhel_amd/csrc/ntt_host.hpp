@@ -38,12 +38,15 @@ struct PassKernel {
     size_t lds_bytes = 0;
 };
 
-// One transform configuration for a degree: the passes in execution order.
+// One transform configuration for a degree: the passes in execution order,
+// and the fused polynomial-product kernel on the same ROW geometry (fn null
+// when the configuration has none).
 struct NttConfig {
     axhel_hip_ntt_config pub{};
     int npass = 0;
     PassKernel fwd[2];
     PassKernel inv[2];
+    PassKernel fused;
 };
 
 // Compiled configurations for log2(degree) (empty for unsupported degrees).

@@ -160,6 +160,77 @@ __device__ __forceinline__ void st_g2(void *p, ulonglong2 v) {
     *(gword2 *)p = x;
 }
 
+// The butterfly chunks of one pass over registers v (chunk-0 layout on entry,
+// last-chunk layout on exit), redistributing through LDS between chunks.
+// Every thread of the workgroup must call it (it holds barriers).
+template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL>
+__device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *lds, uint32_t g, uint32_t tid,
+                                           uint32_t o, uint32_t blo, uint32_t n, const ModConsts &m,
+                                           const ulonglong2 *tw, bool final_inv, const NttJobDev &J) {
+    using Gm = Geom<LOG_G, LOG_E, FWD>;
+    using L = Lds<LOG_G, LOG_E, GPW, COL>;
+    constexpr int E = 1 << LOG_E;
+    constexpr int NC = Gm::NC;
+    static_for<0, NC>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        constexpr int B = Gm::B(c);
+        constexpr int LO = Gm::LO(c);
+        const uint32_t ktid = Gm::kt(c, tid);
+
+        if constexpr (c > 0) {
+            // redistribute: registers in chunk c-1 layout -> LDS -> chunk c layout
+            const uint32_t kp = Gm::kt(c - 1, tid);
+            __syncthreads();
+            static_for<0, E>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                lds[L::idx(g, kp + Gm::kr(c - 1, r))] = v[r];
+            });
+            __syncthreads();
+            static_for<0, E>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                v[r] = lds[L::idx(g, ktid + Gm::kr(c, r))];
+            });
+        }
+
+        // butterflies on local bits LO..LO+B-1 (forward: descending)
+        static_for<0, B>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            constexpr int p = FWD ? LO + B - 1 - s : LO + s;
+            constexpr int bit = 1 << (p - LO);
+            const uint32_t gb = p + blo;  // global bit
+            // thread part of P >> (gb+1); the register part is added per pair
+            uint32_t pt;
+            if constexpr (COL) pt = ktid >> (p + 1);
+            else pt = ((o << LOG_G) | ktid) >> (p + 1);
+            const bool fin = !FWD && final_inv && (p == LOG_G - 1);
+            const uint32_t tbase = FWD ? (n >> (gb + 1)) + pt : n + 1u - (n >> gb) + pt;
+            uint64_t w[E], wq[E];
+            static_for<0, E>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                if constexpr ((r & bit) == 0) {
+                    constexpr int rp = Gm::rep(c, r, p);
+                    constexpr uint32_t pr = Gm::kr(c, r) >> (p + 1);
+                    if constexpr (rp == r) {
+                        if (!fin) {
+                            const ulonglong2 tv = ld_g2(tw + tbase + pr);
+                            w[r] = tv.x;
+                            wq[r] = tv.y;
+                        }
+                    }
+                    if constexpr (FWD) {
+                        fwd_bfly(v[r], v[r | bit], w[rp], wq[rp], m);
+                    } else {
+                        if (fin)
+                            inv_final_bfly(v[r], v[r | bit], J.ninv, J.ninv_q, J.sroot, J.sroot_q, m);
+                        else
+                            inv_bfly(v[r], v[r | bit], w[rp], wq[rp], m);
+                    }
+                }
+            });
+        });
+    });
+}
+
 template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, bool MULTI>
 __global__ void __launch_bounds__(GPW << (LOG_G - LOG_E))
 ntt_pass_kernel(PassArgs a) {
@@ -272,64 +343,7 @@ ntt_pass_kernel(PassArgs a) {
     }
 
     // ---- chunks
-    static_for<0, NC>([&](auto cc) {
-        constexpr int c = decltype(cc)::value;
-        constexpr int B = Gm::B(c);
-        constexpr int LO = Gm::LO(c);
-        const uint32_t ktid = Gm::kt(c, tid);
-
-        if constexpr (c > 0) {
-            // redistribute: registers in chunk c-1 layout -> LDS -> chunk c layout
-            const uint32_t kp = Gm::kt(c - 1, tid);
-            __syncthreads();
-            static_for<0, E>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                lds[L::idx(g, kp + Gm::kr(c - 1, r))] = v[r];
-            });
-            __syncthreads();
-            static_for<0, E>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                v[r] = lds[L::idx(g, ktid + Gm::kr(c, r))];
-            });
-        }
-
-        // butterflies on local bits LO..LO+B-1 (forward: descending)
-        static_for<0, B>([&](auto sc) {
-            constexpr int s = decltype(sc)::value;
-            constexpr int p = FWD ? LO + B - 1 - s : LO + s;
-            constexpr int bit = 1 << (p - LO);
-            const uint32_t gb = p + blo;  // global bit
-            // thread part of P >> (gb+1); the register part is added per pair
-            uint32_t pt;
-            if constexpr (COL) pt = ktid >> (p + 1);
-            else pt = ((o << LOG_G) | ktid) >> (p + 1);
-            const bool fin = !FWD && a.final_inv && (p == LOG_G - 1);
-            const uint32_t tbase = FWD ? (n >> (gb + 1)) + pt : n + 1u - (n >> gb) + pt;
-            uint64_t w[E], wq[E];
-            static_for<0, E>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                if constexpr ((r & bit) == 0) {
-                    constexpr int rp = Gm::rep(c, r, p);
-                    constexpr uint32_t pr = Gm::kr(c, r) >> (p + 1);
-                    if constexpr (rp == r) {
-                        if (!fin) {
-                            const ulonglong2 tv = ld_g2(tw + tbase + pr);
-                            w[r] = tv.x;
-                            wq[r] = tv.y;
-                        }
-                    }
-                    if constexpr (FWD) {
-                        fwd_bfly(v[r], v[r | bit], w[rp], wq[rp], m);
-                    } else {
-                        if (fin)
-                            inv_final_bfly(v[r], v[r | bit], J.ninv, J.ninv_q, J.sroot, J.sroot_q, m);
-                        else
-                            inv_bfly(v[r], v[r | bit], w[rp], wq[rp], m);
-                    }
-                }
-            });
-        });
-    });
+    run_chunks<LOG_G, LOG_E, GPW, FWD, COL>(v, lds, g, tid, o, blo, n, m, tw, a.final_inv != 0, J);
 
     // ---- normalisation folded into the last pass (ntt.cpp:79-118)
     if (a.normalize) {

@@ -1,0 +1,145 @@
+// Fused RNS polynomial product, per limb:
+//   out = InverseNTT( FMAMod( MulMod( NTT(a), NTT(b), 1 ), s, c, 1 ) )
+// the call sequence of tests/public/test_polymul.cpp:13-25 with the FMA step
+// of eltwise-fma-mod.cpp:37-77 in between (normalized transforms, factor 1).
+//
+// The forward transform's last pass and the inverse transform's first pass
+// both act on the low LOG_G index bits of the same contiguous rows, so one
+// kernel does, per row: the forward ROW stages of a and of b, the forward
+// normalisation, MulMod, FMAMod with the addend row, and the inverse ROW
+// stages, all in registers.  The forward pass ends with each thread holding
+// E consecutive words and the inverse pass starts from exactly that layout,
+// so no LDS exchange sits between them.  Every step is the reference's
+// arithmetic on the same words, so the result is bit-identical to running
+// the five calls one after another.
+#pragma once
+
+#include "ntt_kernels.hpp"
+
+namespace axhel_dev {
+
+struct PolyJobDev {
+    const uint64_t *a1;  // a after the forward COL pass (or a itself for a single-pass degree)
+    const uint64_t *b1;  // b likewise
+    const uint64_t *c;   // addend, natural order; null = none
+    uint64_t *out;       // written here; the inverse COL pass then runs in place
+    const ulonglong2 *twf, *twi;
+    uint64_t q;
+    Barrett bc;
+    uint64_t scalar;     // FMA multiplier (ReduceInputNative<1>: as given)
+    uint64_t ninv, ninv_q, sroot, sroot_q;
+};
+
+struct PolyArgs {
+    const PolyJobDev *jobs;
+    uint32_t logn;
+    uint32_t num_limbs;
+    uint32_t groups_per_limb;
+    uint32_t single;  // the degree fits one ROW pass: apply the final stage and normalise here
+    uint32_t vec16;   // addends 16-byte aligned
+};
+
+template <int LOG_G, int LOG_E>
+struct FusedLayout {
+    // the forward pass's last-chunk layout equals the inverse pass's first-chunk layout
+    static constexpr bool ok() {
+        using F = Geom<LOG_G, LOG_E, true>;
+        using I = Geom<LOG_G, LOG_E, false>;
+        for (int r = 0; r < (1 << LOG_E); ++r)
+            if (F::kr(F::NC - 1, r) != I::kr(0, r)) return false;
+        return F::LO(F::NC - 1) == 0 && I::LO(0) == 0;
+    }
+};
+
+template <int LOG_G, int LOG_E, int GPW>
+__global__ void __launch_bounds__(GPW << (LOG_G - LOG_E))
+polymul_row_kernel(PolyArgs a) {
+    static_assert(FusedLayout<LOG_G, LOG_E>::ok(), "forward exit layout must be the inverse entry layout");
+    using Gf = Geom<LOG_G, LOG_E, true>;
+    using Gi = Geom<LOG_G, LOG_E, false>;
+    constexpr int G = 1 << LOG_G;
+    constexpr int E = 1 << LOG_E;
+    constexpr int T = G / E;
+    extern __shared__ uint64_t lds[];
+
+    uint32_t tile = blockIdx.x;
+    {
+        const uint32_t nb = gridDim.x;
+        if ((nb & 7u) == 0) tile = (tile & 7u) * (nb >> 3) + (tile >> 3);
+    }
+    const uint32_t t = threadIdx.x;
+    const uint32_t tid = t % T, g = t / T;
+    const uint32_t gid = tile * GPW + g;
+    const uint32_t limb = (tile * GPW) / a.groups_per_limb;  // uniform (groups_per_limb % GPW == 0)
+    const uint32_t o = gid - limb * a.groups_per_limb;
+    const PolyJobDev P = a.jobs[limb];
+    const ModConsts m = mod_consts(P.q);
+    const uint32_t n = 1u << a.logn;
+    const uint32_t row = o << LOG_G;
+
+    NttJobDev fin{};
+    fin.ninv = P.ninv;
+    fin.ninv_q = P.ninv_q;
+    fin.sroot = P.sroot;
+    fin.sroot_q = P.sroot_q;
+
+    uint64_t va[E], vb[E];
+    const uint32_t k0 = Gf::kt(0, tid);
+    static_for<0, E>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        va[r] = ld_g(P.a1 + (row | (k0 + Gf::kr(0, r))));
+    });
+    run_chunks<LOG_G, LOG_E, GPW, true, false>(va, lds, g, tid, o, 0, n, m, P.twf, false, fin);
+    static_for<0, E>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        vb[r] = ld_g(P.b1 + (row | (k0 + Gf::kr(0, r))));
+    });
+    __syncthreads();  // the LDS tile is reused by b's pass
+    run_chunks<LOG_G, LOG_E, GPW, true, false>(vb, lds, g, tid, o, 0, n, m, P.twf, false, fin);
+
+    // thread holds words row + kl + r, r in [0, E)
+    const uint32_t kl = Gf::kt(Gf::NC - 1, tid);
+    uint64_t cv[E];
+    if (P.c) {
+        if (a.vec16) {
+            static_for<0, E / 2>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const ulonglong2 x2 = ld_g2(P.c + (row | kl) + 2 * i);
+                cv[2 * i] = x2.x;
+                cv[2 * i + 1] = x2.y;
+            });
+        } else {
+            static_for<0, E>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                cv[r] = ld_g(P.c + (row | kl) + r);
+            });
+        }
+    }
+    static_for<0, E>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        // NTT normalisation (ntt.cpp:79-91), then MulMod and FMAMod at factor 1
+        const uint64_t x = csub_n(csub_n(va[r], m.q2, m.nq2), m.q, m.nq);
+        const uint64_t y = csub_n(csub_n(vb[r], m.q2, m.nq2), m.q, m.nq);
+        const uint64_t prod = barrett_mul(x, y, P.bc);
+        uint64_t d = barrett_mul(prod, P.scalar, P.bc);
+        if (P.c) {
+            d += cv[r];
+            if (d >= m.q) d -= m.q;
+        }
+        va[r] = d;
+    });
+    run_chunks<LOG_G, LOG_E, GPW, false, false>(va, lds, g, tid, o, 0, n, m, P.twi, a.single != 0, fin);
+    if (a.single) {
+        static_for<0, E>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            va[r] = csub_n(va[r], m.q, m.nq);
+        });
+    }
+    const uint32_t ke = Gi::kt(Gi::NC - 1, tid);
+    static_for<0, E>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        st_g(P.out + (row | (ke + Gi::kr(Gi::NC - 1, r))), va[r]);
+    });
+}
+
+}  // namespace axhel_dev

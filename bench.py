@@ -95,6 +95,98 @@ def eltwise_leg(ax, torch, src, dst, q, reps):
     return out
 
 
+def polymul_leg(ax, torch, params, polys=128, limbs=32, logn=15, reps=5):
+    """Fused RNS polynomial product, 128 polys x 32 limbs x N=2^15: NTT(a),
+    NTT(b), MulMod, FMAMod(scalar, addend), InverseNTT per limb; fused
+    pipeline vs the same five calls as batched kernels."""
+    n = 1 << logn
+    qs = params.rns_primes(n, limbs)
+    plans = [ax.NTT(n, q, params.find_primitive_2n_root(n, q)) for q in qs]
+    g = torch.Generator(device="cuda").manual_seed(3)
+    shape = (polys, limbs, n)
+    A = torch.empty(shape, dtype=torch.int64, device="cuda")
+    B = torch.empty_like(A)
+    C = torch.empty_like(A)
+    for l, q in enumerate(qs):
+        A[:, l] = torch.randint(0, q, (polys, n), generator=g, device="cuda", dtype=torch.int64)
+        B[:, l] = torch.randint(0, q, (polys, n), generator=g, device="cuda", dtype=torch.int64)
+        C[:, l] = torch.randint(0, q, (polys, n), generator=g, device="cuda", dtype=torch.int64)
+    out = torch.empty_like(A)
+    scr = torch.empty_like(A)
+    res = {"workload": f"{polys} polys x {limbs} limbs x N={n}", "limb_products": polys * limbs}
+    outs = {}
+    for fused in (True, False):
+        jobs = [(plans[l], out[p, l], A[p, l], B[p, l], C[p, l], 12345 + l, scr[p, l])
+                for p in range(polys) for l in range(limbs)]
+        pm = ax.Polymul(jobs, fused=fused)
+        pm.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            pm.launch()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        key = "fused" if fused else "unfused"
+        res[key + "_ms"] = ms
+        res[key + "_limb_products_per_s"] = polys * limbs / (ms * 1e-3)
+        outs[key] = out.clone()
+        pm.close()
+    res["fused_equals_unfused"] = bool(torch.equal(outs["fused"], outs["unfused"]))
+    return res
+
+
+def uneven_leg(ax, torch, params, jobs=512, reps=5, seed=0):
+    """512 jobs in one graph replay: N = 2^(12+U{0..5}), limbs U{1..40},
+    moduli of 30..60 bits, forward and inverse mixed."""
+    rng = np.random.default_rng(seed)
+    pools = {}
+    for logn in range(12, 18):
+        n = 1 << logn
+        qs, seen = [], {}
+        for i in range(40):
+            bits = 30 + (i * 31) // 40
+            k = seen.get(bits, 0)
+            seen[bits] = k + 1
+            qs.append(params.ntt_primes(n, bits, k + 1)[k])
+        pools[logn] = [ax.NTT(n, q, params.find_primitive_2n_root(n, q)) for q in qs]
+    spec = []
+    total_words = 0
+    for j in range(jobs):
+        logn = int(rng.integers(12, 18))
+        limbs = int(rng.integers(1, 41))
+        inverse = bool(rng.integers(0, 2))
+        spec.append((logn, limbs, inverse, int(rng.integers(0, 40))))
+        total_words += limbs << logn
+    src = torch.randint(0, 1 << 29, (total_words,), device="cuda", dtype=torch.int64)
+    dst = torch.empty_like(src)
+    blist = []
+    off = 0
+    limbs_total = 0
+    for logn, limbs, inverse, k in spec:
+        n = 1 << logn
+        for l in range(limbs):
+            blist.append((pools[logn][(k + l) % 40], dst[off:off + n], src[off:off + n], inverse, 1))
+            off += n
+        limbs_total += limbs
+    batch = ax.Batch(blist)
+    batch.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.launch()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.launch_eager()
+    torch.cuda.synchronize()
+    ems = (time.perf_counter() - t0) * 1e3 / reps
+    batch.close()
+    return {"jobs": jobs, "limb_transforms": limbs_total, "words": total_words, "graph_ms": ms,
+            "eager_ms": ems, "limbs_per_s": limbs_total / (ms * 1e-3),
+            "coeff_per_s": total_words / (ms * 1e-3)}
+
+
 def serve_leg(ax, seconds):
     """Serving loop under Poisson load (native driver axhel_serve_bench):
     capacity from a closed loop, then p50/p99 at 0.5x and 0.8x of it."""
@@ -121,6 +213,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eltwise", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-polymul", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run one 2^27-word EltwiseReduceMod(4->2) (1 GiB in, 1 GiB out): "
@@ -258,6 +352,16 @@ def main():
         elt = eltwise_leg(ax, torch, src, dst, primes[0], reps=max(3, min(10, args.steps)))
         log("eltwise GB/s:", {k: round(v["GB/s"], 1) for k, v in elt.items()})
 
+    del batch, ibatch
+    pmul = None
+    if not args.no_polymul:
+        pmul = polymul_leg(ax, torch, params)
+        log("polymul:", json.dumps(pmul))
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_leg(ax, torch, params)
+        log("uneven batch:", json.dumps(uneven))
+
     serving = None
     if not args.no_serve:
         serving = serve_leg(ax, args.serve_seconds)
@@ -297,6 +401,8 @@ def main():
                      "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved_gbs / HBM_PEAK_GBS}},
         "eltwise": elt,
+        "polymul": pmul,
+        "uneven_batch": uneven,
         "serving": serving,
         "cpu_baseline": cpu,
     }

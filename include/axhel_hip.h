@@ -154,6 +154,36 @@ axhel_hip_status axhel_hip_ntt_batch_launch_kernel(axhel_hip_batch *batch, int i
 /* One-shot batch: create, launch eagerly on `stream`, synchronize, destroy. */
 axhel_hip_status axhel_hip_ntt_batch_run(const axhel_hip_ntt_job *jobs, size_t count, void *stream);
 
+/* ------------------------------------------------------ polynomial product */
+
+/* One limb of an RNS polynomial product:
+     result = InverseNTT(FMAMod(MulMod(NTT(a), NTT(b), 1), scalar, addend, 1))
+   with normalized transforms: the call sequence of
+   tests/public/test_polymul.cpp:13-25 (ComputeForward x2, EltwiseMulMod,
+   ComputeInverse) with EltwiseFMAMod (eltwise-fma-mod.hpp:15) before the
+   inverse.  Device pointers, N words each; addend may be NULL; scratch is N
+   words the library may overwrite.  No reference counterpart as one call. */
+typedef struct axhel_hip_polymul_job {
+    const axhel_hip_plan *plan;
+    uint64_t *result;
+    const uint64_t *a;
+    const uint64_t *b;
+    const uint64_t *addend;
+    uint64_t scalar;
+    uint64_t *scratch;
+} axhel_hip_polymul_job;
+
+typedef struct axhel_hip_polymul axhel_hip_polymul;
+
+/* fused != 0: forward COL passes of a and b, one fused kernel (forward ROW
+   stages of both, MulMod, FMAMod, inverse ROW stages), inverse COL pass.
+   fused == 0: the five calls as batched kernels, in the reference's order.
+   Both give the same bits. */
+axhel_hip_status axhel_hip_polymul_create(const axhel_hip_polymul_job *jobs, size_t count, int fused,
+                                          axhel_hip_polymul **pm);
+axhel_hip_status axhel_hip_polymul_launch(axhel_hip_polymul *pm, void *stream);
+axhel_hip_status axhel_hip_polymul_destroy(axhel_hip_polymul *pm);
+
 /* ---------------------------------------------------------------- serving */
 
 /* Request loop with continuous batching.  No reference counterpart: the

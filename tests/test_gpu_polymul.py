@@ -1,0 +1,94 @@
+"""Batched RNS polynomial product on the GPU (fused and as the plain call
+sequence), bit-exact against the oracle running the reference's calls:
+ComputeForward(a), ComputeForward(b), EltwiseMulMod(F=1), EltwiseFMAMod(F=1),
+ComputeInverse (tests/public/test_polymul.cpp:13-25 plus the FMA step)."""
+import numpy as np
+import pytest
+
+import axhel_amd as ax
+from axhel_amd import params
+from oracle import oracle as O
+from tests.helpers import dev, empty_dev, host, torch_mod
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_polymul(a, b, c, s, t, q):
+    A = O.compute_forward(a, t, 1, 1)
+    B = O.compute_forward(b, t, 1, 1)
+    C = O.mul_mod(A, B, q, 1)
+    D = O.fma_mod(C, s, c, q, 1)
+    return O.compute_inverse(D, t, 1, 1)
+
+
+def run_case(n, limbs, fused, with_addend, seed):
+    torch = torch_mod()
+    rng = np.random.default_rng(seed)
+    qs = params.rns_primes(n, limbs)
+    psis = [params.find_primitive_2n_root(n, q) for q in qs]
+    plans = [ax.NTT(n, q, p) for q, p in zip(qs, psis)]
+    A = np.stack([params.uniform(rng, n, q) for q in qs])
+    B = np.stack([params.uniform(rng, n, q) for q in qs])
+    C = np.stack([params.uniform(rng, n, q) for q in qs])
+    S = [int(params.uniform(rng, 1, q)[0]) for q in qs]
+    da, db, dc = dev(A.ravel()).view(limbs, n), dev(B.ravel()).view(limbs, n), dev(C.ravel()).view(limbs, n)
+    out = empty_dev(limbs * n).view(limbs, n)
+    scr = empty_dev(limbs * n).view(limbs, n)
+    jobs = [(plans[l], out[l], da[l], db[l], dc[l] if with_addend else None, S[l], scr[l]) for l in range(limbs)]
+    pm = ax.Polymul(jobs, fused=fused)
+    pm.launch()
+    got = host(out).reshape(limbs, n)
+    for l in range(limbs):
+        t = O.Tables(n, qs[l], psis[l])
+        want = oracle_polymul(A[l], B[l], C[l] if with_addend else None, S[l], t, qs[l])
+        assert np.array_equal(got[l], want), (n, l, fused, with_addend)
+    # inputs untouched
+    assert np.array_equal(host(da).reshape(limbs, n), A)
+    assert np.array_equal(host(db).reshape(limbs, n), B)
+    pm.close()
+
+
+@pytest.mark.parametrize("logn", [10, 12, 13, 14, 15, 16, 17])
+@pytest.mark.parametrize("fused", [True, False])
+def test_polymul_matches_call_sequence(logn, fused):
+    run_case(1 << logn, 3, fused, True, logn)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_polymul_null_addend(fused):
+    run_case(1 << 15, 2, fused, False, 7)
+
+
+def test_polymul_every_config_n32768():
+    n = 1 << 15
+    try:
+        for ci in range(ax.config_count(n)):
+            ax.config_select(n, ci)
+            run_case(n, 2, True, True, 100 + ci)
+    finally:
+        ax.config_select(n, -1)
+
+
+def test_polymul_is_the_negacyclic_product():
+    """scalar 1, no addend: the result is a*b mod (X^N + 1)."""
+    torch = torch_mod()
+    n = 1 << 12
+    q = params.find_ntt_prime(n, 60)
+    psi = params.find_primitive_2n_root(n, q)
+    ntt = ax.NTT(n, q, psi)
+    rng = np.random.default_rng(3)
+    a = np.zeros(n, dtype=np.uint64)
+    b = params.uniform(rng, n, q)
+    a[0], a[1], a[n - 1] = 5, 1, q - 2  # 5 + x - 2 x^(N-1)
+    out, scr = empty_dev(n), empty_dev(n)
+    pm = ax.Polymul([(ntt, out, dev(a), dev(b), None, 1, scr)], fused=True)
+    pm.launch()
+    got = host(out)
+    bi = [int(x) for x in b]
+    want = []
+    for k in range(n):
+        v = 5 * bi[k]
+        v += bi[k - 1] if k >= 1 else -bi[n - 1]          # x * b: shift with negacyclic wrap
+        v += 2 * bi[k + 1] if k + 1 < n else -2 * bi[0]  # -2 x^(N-1) * b: x^(N-1+j) = -x^(j-1)
+        want.append(v % q)
+    assert got.tolist() == want
