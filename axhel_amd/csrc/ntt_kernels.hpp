@@ -144,10 +144,20 @@ typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) uint64_t gword;
 typedef __attribute__((address_space(1))) u64x2 gword2;
 
-__device__ __forceinline__ uint64_t ld_g(const uint64_t *p) { return *(const gword *)p; }
-__device__ __forceinline__ void st_g(uint64_t *p, uint64_t v) { *(gword *)p = v; }
+// Coefficient traffic streams through once per pass: it is loaded and stored
+// non-temporally so it does not evict the twiddle tables from L2 (twiddles,
+// read through ld_g2, stay cacheable).
+__device__ __forceinline__ uint64_t ld_g(const uint64_t *p) { return __builtin_nontemporal_load((const gword *)p); }
+__device__ __forceinline__ void st_g(uint64_t *p, uint64_t v) { __builtin_nontemporal_store(v, (gword *)p); }
 __device__ __forceinline__ ulonglong2 ld_g2(const void *p) {
     const u64x2 x = *(const gword2 *)p;
+    ulonglong2 r;
+    r.x = x.x;
+    r.y = x.y;
+    return r;
+}
+__device__ __forceinline__ ulonglong2 ld_nt2(const void *p) {
+    const u64x2 x = __builtin_nontemporal_load((const gword2 *)p);
     ulonglong2 r;
     r.x = x.x;
     r.y = x.y;
@@ -157,7 +167,7 @@ __device__ __forceinline__ void st_g2(void *p, ulonglong2 v) {
     u64x2 x;
     x.x = v.x;
     x.y = v.y;
-    *(gword2 *)p = x;
+    __builtin_nontemporal_store(x, (gword2 *)p);
 }
 
 // The butterfly chunks of one pass over registers v (chunk-0 layout on entry,
@@ -303,7 +313,7 @@ ntt_pass_kernel(PassArgs a) {
 #pragma unroll
                 for (int i = 0; i < E / 2; ++i) {
                     const uint32_t e = 2u * (t + (uint32_t)i * NT);
-                    const ulonglong2 x2 = ld_g2(ts + e);
+                    const ulonglong2 x2 = ld_nt2(ts + e);
                     const uint32_t ge = e >> LOG_G, ke = e & (G - 1);
                     lds[L::idx(ge, ke)] = x2.x;
                     lds[L::idx(ge, ke + 1)] = x2.y;

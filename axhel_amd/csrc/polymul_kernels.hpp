@@ -104,7 +104,7 @@ polymul_row_kernel(PolyArgs a) {
         if (a.vec16) {
             static_for<0, E / 2>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
-                const ulonglong2 x2 = ld_g2(P.c + (row | kl) + 2 * i);
+                const ulonglong2 x2 = ld_nt2(P.c + (row | kl) + 2 * i);
                 cv[2 * i] = x2.x;
                 cv[2 * i + 1] = x2.y;
             });
