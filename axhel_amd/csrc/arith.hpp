@@ -36,21 +36,60 @@ __device__ __forceinline__ uint64_t shoup(uint64_t y, uint64_t w, uint64_t wq, u
     return y * w - qhat * q;
 }
 
-// The same Shoup product, spelled for the gfx950 VALU.  nq = 2^64 - q, so
-// y*w - qhat*q == y*w + qhat*nq (mod 2^64); the two low products then chain
-// through v_mad_u64_u32 accumulators and the four cross terms land in the
-// high word with two 3-input adds, with no 64-bit carry chain.  Same value
-// as shoup() for every input.
+// gfx950 VALU forms.  Two inline-assembly pieces expose what the compiler
+// does not: the carry-out of v_mad_u64_u32 and a select driven by the carry
+// of a 64-bit add.  A VALU that writes an SGPR (a carry or lane mask) needs
+// two wait states before a VALU reads that SGPR; the compiler does not see
+// inside these blocks, so each block carries its own s_nop and keeps every
+// SGPR it writes to itself.  All operands are VGPRs, so the forms hold for
+// per-lane as well as uniform values.
+
+// u + carry * 2^64 = a * b + t; carry returned as 0 or 1.
+__device__ __forceinline__ uint64_t mad_u64_carry(uint32_t a, uint32_t b, uint64_t t, uint32_t &carry) {
+    uint64_t u, c;
+    asm("v_mad_u64_u32 %0, %1, %3, %4, %5\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %2, 0, 1, %1"
+        : "=v"(u), "=s"(c), "=v"(carry)
+        : "v"(a), "v"(b), "v"(t));
+    return u;
+}
+
+__device__ __forceinline__ uint32_t add3_u32(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_add3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Exact hi64(a * b) in six VALU instructions: the middle column
+// a0*b1 + a1*b0 + hi(a0*b0) can reach 2^65, and its carry is taken from the
+// second v_mad_u64_u32 instead of being rebuilt from zero-extended halves.
+__device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t t = (uint64_t)a1 * b0 + __umulhi(a0, b0);  // < 2^64
+    uint32_t carry;
+    const uint64_t u = mad_u64_carry(a0, b1, t, carry);
+    return (uint64_t)a1 * b1 + ((u >> 32) | ((uint64_t)carry << 32));
+}
+
+// The Shoup product of shoup(), spelled for the gfx950 VALU.  nq = 2^64 - q,
+// so y*w - qhat*q == y*w + qhat*nq (mod 2^64): the two low products chain
+// through one v_mad_u64_u32 accumulator and the four cross terms (two per
+// v_mul_lo/v_mad pair) land in the high word with one 3-input add.  Same
+// value as shoup() for every input.
 __device__ __forceinline__ uint64_t shoup_n(uint64_t y, uint64_t w, uint64_t wq, uint64_t nq) {
-    const uint64_t qhat = __umul64hi(y, wq);
+    const uint64_t qhat = mulhi64(y, wq);
     const uint32_t yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
     const uint32_t wl = (uint32_t)w, wh = (uint32_t)(w >> 32);
     const uint32_t ql = (uint32_t)qhat, qh = (uint32_t)(qhat >> 32);
     const uint32_t nl = (uint32_t)nq, nh = (uint32_t)(nq >> 32);
-    uint64_t t = (uint64_t)yl * wl + (uint64_t)ql * nl;
-    const uint32_t cross = yl * wh + yh * wl + ql * nh + qh * nl;
-    t += (uint64_t)cross << 32;
-    return t;
+    uint64_t p = (uint64_t)yl * wl;
+    p = (uint64_t)ql * nl + p;
+    const uint32_t c1 = yl * wh + yh * wl;
+    const uint32_t c2 = ql * nh + qh * nl;
+    const uint32_t hi = add3_u32((uint32_t)(p >> 32), c1, c2);
+    return (p & 0xffffffffull) | ((uint64_t)hi << 32);
 }
 
 // Per-limb constants of the butterflies: q, 2q and their negations mod 2^64.
@@ -67,10 +106,22 @@ __device__ __forceinline__ ModConsts mod_consts(uint64_t q) {
     return m;
 }
 
-// csub(x, b) given nb = -b: one 64-bit add (v_lshl_add_u64) and a select.
+// csub(x, b) given nb = 2^64 - b (b > 0): x + nb carries out exactly when
+// x >= b, and that carry selects between x + nb and x (four VALU
+// instructions).
 __device__ __forceinline__ uint64_t csub_n(uint64_t x, uint64_t b, uint64_t nb) {
-    const uint64_t d = x + nb;
-    return x >= b ? d : x;
+    (void)b;
+    uint32_t lo, hi;
+    uint64_t c;
+    asm("v_add_co_u32_e64 %0, %2, %3, %5\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %1, %2, %4, %6, %2\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %0, %3, %0, %2\n\t"
+        "v_cndmask_b32_e64 %1, %4, %1, %2"
+        : "=&v"(lo), "=&v"(hi), "=&s"(c)
+        : "v"((uint32_t)x), "v"((uint32_t)(x >> 32)), "v"((uint32_t)nb), "v"((uint32_t)(nb >> 32)));
+    return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 // Harvey forward (Cooley-Tukey) butterfly, ntt-internal.hpp:20-31.
