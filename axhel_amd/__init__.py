@@ -56,7 +56,8 @@ class NttJob(ctypes.Structure):
 class NttConfig(ctypes.Structure):
     """axhel_hip_ntt_config (include/axhel_hip.h)."""
     _fields_ = [("log_elems", ctypes.c_int32), ("row_groups", ctypes.c_int32),
-                ("col_groups", ctypes.c_int32), ("log_col", ctypes.c_int32)]
+                ("col_groups", ctypes.c_int32), ("log_col", ctypes.c_int32),
+                ("log_elems_col", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -151,6 +152,14 @@ def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run `python -m axhel_amd.build` "
                           "(or __graft_entry__.build())")
+    # PyTorch-ROCm ships its own libamdhip64 / libhsa-runtime64.  Loading
+    # torch first lets this library bind to those (same soname), so the
+    # process holds one HIP runtime; loaded the other way round, a second
+    # runtime initialises the device and torch then finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -108,6 +108,7 @@ NttConfig single() {
     c.pub.row_groups = GPW;
     c.pub.col_groups = 0;
     c.pub.log_col = 0;
+    c.pub.log_elems_col = 0;
     c.npass = 1;
     c.fwd[0] = pk<LG, LE, GPW, false, (GPW > 1), true>();
     c.inv[0] = pk<LG, LE, GPW, false, (GPW > 1), false>();
@@ -123,6 +124,7 @@ NttConfig two() {
     c.pub.row_groups = GR;
     c.pub.col_groups = GC;
     c.pub.log_col = LC;
+    c.pub.log_elems_col = LEC;
     c.npass = 2;
     c.fwd[0] = pk<LC, LEC, GC, true, false, true>();
     c.fwd[1] = pk<LR, LER, GR, false, false, true>();
@@ -147,10 +149,15 @@ std::vector<std::vector<NttConfig>> build_registry() {
     r[11] = {single<11, 4, 2>(), single<11, 3, 1>()};
     r[12] = {single<12, 4, 1>(), single<12, 3, 1>()};
     r[13] = {single<13, 4, 1>(), single<13, 3, 1>()};
-    r[14] = {two<7, 4, 32, 7, 4, 32>(), two<7, 4, 16, 7, 4, 16>()};
-    r[15] = {two<8, 4, 16, 7, 4, 32>(), two<7, 4, 32, 8, 4, 16>()};
-    r[16] = {two<8, 4, 16, 8, 4, 16>(), two<8, 4, 8, 8, 4, 8>(), two<8, 3, 16, 8, 3, 8>()};
-    r[17] = {two<9, 3, 8, 8, 4, 16>(), two<8, 4, 16, 9, 3, 8>()};
+    r[14] = {two<7, 4, 32, 7, 4, 32>(), two<7, 4, 16, 7, 4, 16>(), two<7, 3, 16, 7, 3, 16>(),
+             two<7, 3, 32, 7, 3, 16>()};
+    r[15] = {two<8, 4, 16, 7, 4, 32>(), two<7, 4, 32, 8, 4, 16>(), two<8, 3, 16, 7, 3, 16>(),
+             two<7, 3, 16, 8, 3, 8>()};
+    r[16] = {two<8, 4, 16, 8, 4, 16>(), two<8, 4, 8, 8, 4, 8>(), two<8, 3, 16, 8, 3, 8>(),
+             two<8, 3, 16, 8, 4, 8>(),  two<8, 4, 16, 8, 3, 8>(), two<8, 3, 8, 8, 3, 8>(),
+             two<8, 3, 16, 8, 3, 16>(), two<7, 3, 16, 9, 3, 4>(), two<9, 3, 8, 7, 3, 16>()};
+    r[17] = {two<9, 3, 8, 8, 4, 16>(), two<8, 4, 16, 9, 3, 8>(), two<9, 3, 8, 8, 3, 8>(),
+             two<8, 3, 16, 9, 3, 4>()};
     return r;
 }
 

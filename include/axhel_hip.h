@@ -281,12 +281,14 @@ axhel_hip_status axhel_hip_server_reset_stats(axhel_hip_server *server);
    log_col bits followed by a ROW pass over the rest (inverse: reversed).
    log_elems = log2 words per thread in the ROW pass, i.e. the butterfly
    stages done in registers between LDS exchanges (the radix); row_groups /
-   col_groups = groups (rows / columns) per workgroup. */
+   col_groups = groups (rows / columns) per workgroup; log_elems_col = the
+   COL pass's radix. */
 typedef struct axhel_hip_ntt_config {
     int32_t log_elems;
     int32_t row_groups;
-    int32_t col_groups; /* 0 for a single pass */
-    int32_t log_col;    /* 0 for a single pass */
+    int32_t col_groups;    /* 0 for a single pass */
+    int32_t log_col;       /* 0 for a single pass */
+    int32_t log_elems_col; /* 0 for a single pass */
 } axhel_hip_ntt_config;
 
 /* Number of compiled configurations for a degree, and the i-th of them. */

@@ -3,6 +3,8 @@ bit-exact against the oracle, lazy outputs included (tests/public/test_ntt.cpp
 restated, extended to every compiled kernel configuration and N up to 2^17)."""
 import json
 import os
+import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -146,3 +148,14 @@ def test_invalid_plans_are_rejected():
         ax.NTT(12, 97, 3)  # not a power of two
     with pytest.raises(ax.AxhelError):
         ax.NTT(1 << 18, 0x3FFFFFFFFFF00001, 3)  # above the supported degree
+
+
+def test_library_loaded_before_torch_leaves_torch_a_gpu():
+    """Creating a plan (HIP calls) before torch is imported must not leave the
+    process with a second HIP runtime that hides the GPU from torch."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys; import axhel_amd as ax; from axhel_amd import params; n = 1024; "
+            "q = params.find_ntt_prime(n, 50); p = ax.NTT(n, q, params.find_primitive_2n_root(n, q)); "
+            "assert 'torch' in sys.modules; import torch; assert torch.cuda.is_available(); print('ok')")
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stderr[-2000:]
