@@ -73,23 +73,48 @@ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
     return (uint64_t)a1 * b1 + ((u >> 32) | ((uint64_t)carry << 32));
 }
 
-// The Shoup product of shoup(), spelled for the gfx950 VALU.  nq = 2^64 - q,
-// so y*w - qhat*q == y*w + qhat*nq (mod 2^64): the two low products chain
-// through one v_mad_u64_u32 accumulator and the four cross terms (two per
-// v_mul_lo/v_mad pair) land in the high word with one 3-input add.  Same
-// value as shoup() for every input.
-__device__ __forceinline__ uint64_t shoup_n(uint64_t y, uint64_t w, uint64_t wq, uint64_t nq) {
+// acc + shoup(y, w, wq, q) mod 2^64, spelled for the gfx950 VALU.  With
+// nq = 2^64 - q, y*w - qhat*q == y*w + qhat*nq (mod 2^64): the two low
+// products chain through one v_mad_u64_u32 accumulator that starts at acc,
+// and the four cross terms (two per v_mul_lo/v_mad pair) land in the high
+// word with one 3-input add.  acc = 0 gives shoup() itself.
+__device__ __forceinline__ uint64_t shoup_acc(uint64_t acc, uint64_t y, uint64_t w, uint64_t wq, uint64_t nq) {
     const uint64_t qhat = mulhi64(y, wq);
     const uint32_t yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
     const uint32_t wl = (uint32_t)w, wh = (uint32_t)(w >> 32);
     const uint32_t ql = (uint32_t)qhat, qh = (uint32_t)(qhat >> 32);
     const uint32_t nl = (uint32_t)nq, nh = (uint32_t)(nq >> 32);
-    uint64_t p = (uint64_t)yl * wl;
+    uint64_t p = (uint64_t)yl * wl + acc;
     p = (uint64_t)ql * nl + p;
     const uint32_t c1 = yl * wh + yh * wl;
     const uint32_t c2 = ql * nh + qh * nl;
     const uint32_t hi = add3_u32((uint32_t)(p >> 32), c1, c2);
     return (p & 0xffffffffull) | ((uint64_t)hi << 32);
+}
+
+__device__ __forceinline__ uint64_t shoup_n(uint64_t y, uint64_t w, uint64_t wq, uint64_t nq) {
+    return shoup_acc(0, y, w, wq, nq);
+}
+
+// a - b mod 2^64 as one borrow chain.  Left to itself the compiler splits a
+// difference whose subtrahend was assembled from two halves (as shoup_acc's
+// result is) into several partial subtractions.
+__device__ __forceinline__ uint64_t sub64(uint64_t a, uint64_t b) {
+    uint32_t lo, hi;
+    uint64_t c;
+    asm("v_sub_co_u32_e64 %0, %2, %3, %5\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %1, %2, %4, %6, %2"
+        : "=&v"(lo), "=v"(hi), "=&s"(c)
+        : "v"((uint32_t)a), "v"((uint32_t)(a >> 32)), "v"((uint32_t)b), "v"((uint32_t)(b >> 32)));
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// 2a + b in one v_lshl_add_u64 (the compiler otherwise shifts and adds).
+__device__ __forceinline__ uint64_t twice_plus(uint64_t a, uint64_t b) {
+    uint64_t r;
+    asm("v_lshl_add_u64 %0, %1, 1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
 }
 
 // Per-limb constants of the butterflies: q, 2q and their negations mod 2^64.
@@ -124,12 +149,16 @@ __device__ __forceinline__ uint64_t csub_n(uint64_t x, uint64_t b, uint64_t nb) 
     return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
-// Harvey forward (Cooley-Tukey) butterfly, ntt-internal.hpp:20-31.
+// Harvey forward (Cooley-Tukey) butterfly, ntt-internal.hpp:20-31:
+// x' = csub(x, 2q), t = shoup(y), x = x' + t, y = x' + 2q - t.  The sum
+// comes out of the Shoup accumulator and the difference as
+// (2x' + 2q) - (x' + t); both are below 4q, so the wrap-around arithmetic
+// gives the reference's values exactly.
 __device__ __forceinline__ void fwd_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wq, const ModConsts &m) {
     const uint64_t xh = csub_n(x, m.q2, m.nq2);
-    const uint64_t t = shoup_n(y, w, wq, m.nq);
-    x = xh + t;
-    y = xh + m.q2 - t;
+    const uint64_t sum = shoup_acc(xh, y, w, wq, m.nq);
+    y = sub64(twice_plus(xh, m.q2), sum);
+    x = sum;
 }
 
 // Harvey inverse (Gentleman-Sande) butterfly, ntt-internal.hpp:37-49.

@@ -83,7 +83,7 @@ PassKernel pk() {
     k.col = COL;
     k.multi = MULTI;
     k.threads = GPW << (LG - LE);
-    k.lds_bytes = (size_t)axhel_dev::Lds<LG, LE, GPW, COL>::WORDS * sizeof(uint64_t);
+    k.lds_bytes = (size_t)axhel_dev::Lds<LG, LE, GPW, COL>::TOTAL_WORDS * sizeof(uint64_t);
     return k;
 }
 

@@ -132,6 +132,10 @@ struct Lds {
     // hit adjacent words.
     static constexpr int ROW_STRIDE = G + (G >> LOG_E);
     static constexpr int WORDS = COL ? GPW * G : GPW * ROW_STRIDE;
+    // COL passes stage the pass's G - 1 twiddle pairs after the data tile
+    // (WORDS is even, so the pairs stay 16-byte aligned).
+    static constexpr int TW_WORDS = COL ? 2 * (G - 1) : 0;
+    static constexpr int TOTAL_WORDS = WORDS + TW_WORDS;
     __device__ static uint32_t idx(uint32_t g, uint32_t k) {
         if constexpr (COL) return k * GPW + g;
         else return g * ROW_STRIDE + k + (k >> LOG_E);
@@ -170,13 +174,52 @@ __device__ __forceinline__ void st_g2(void *p, ulonglong2 v) {
     __builtin_nontemporal_store(x, (gword2 *)p);
 }
 
+// Twiddles of a pass.  On local bit p (j = LOG_G - 1 - p) a pass uses 2^j
+// consecutive table entries, from
+//     forward:  (N >> (gb+1)) + (o << j)          gb = p + blo, the global bit
+//     inverse:  N + 1 - (N >> gb) + (o << j)
+// where the (o << j) term (the row) appears in ROW passes only; lp in
+// [0, 2^j) selects the entry.  TwGlobal reads the table in HBM/L2; TwLds reads
+// a copy staged in LDS, entry (j, lp) at
+//     forward:  2^j - 1 + lp        inverse:  2^LOG_G - 2^(j+1) + lp
+// (both orders cover [0, 2^LOG_G - 1) once).  In a COL pass both orders
+// match the table's own: the copy is entries [base, base + 2^LOG_G - 1) with
+// base = 1 (forward) or N + 1 - 2^LOG_G (inverse).
+template <int LOG_G, bool FWD>
+__device__ __forceinline__ constexpr uint32_t tw_lds_index(int j, uint32_t lp) {
+    return (FWD ? (1u << j) - 1u : (1u << LOG_G) - (2u << j)) + lp;
+}
+
+template <int LOG_G, bool FWD, bool COL>
+struct TwGlobal {
+    const ulonglong2 *tw;
+    uint32_t n, blo, o;
+    template <int P>
+    __device__ __forceinline__ ulonglong2 get(uint32_t lp) const {
+        constexpr int j = LOG_G - 1 - P;
+        const uint32_t gb = P + blo;
+        uint32_t base = FWD ? (n >> (gb + 1)) : n + 1u - (n >> gb);
+        if constexpr (!COL) base += o << j;
+        return ld_g2(tw + base + lp);
+    }
+};
+
+template <int LOG_G, bool FWD>
+struct TwLds {
+    const ulonglong2 *t;
+    template <int P>
+    __device__ __forceinline__ ulonglong2 get(uint32_t lp) const {
+        constexpr int j = LOG_G - 1 - P;
+        return t[tw_lds_index<LOG_G, FWD>(j, lp)];
+    }
+};
+
 // The butterfly chunks of one pass over registers v (chunk-0 layout on entry,
 // last-chunk layout on exit), redistributing through LDS between chunks.
 // Every thread of the workgroup must call it (it holds barriers).
-template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL>
+template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, typename TW>
 __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *lds, uint32_t g, uint32_t tid,
-                                           uint32_t o, uint32_t blo, uint32_t n, const ModConsts &m,
-                                           const ulonglong2 *tw, bool final_inv, const NttJobDev &J) {
+                                           const ModConsts &m, const TW &tws, bool final_inv, const NttJobDev &J) {
     using Gm = Geom<LOG_G, LOG_E, FWD>;
     using L = Lds<LOG_G, LOG_E, GPW, COL>;
     constexpr int E = 1 << LOG_E;
@@ -207,13 +250,9 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
             constexpr int s = decltype(sc)::value;
             constexpr int p = FWD ? LO + B - 1 - s : LO + s;
             constexpr int bit = 1 << (p - LO);
-            const uint32_t gb = p + blo;  // global bit
-            // thread part of P >> (gb+1); the register part is added per pair
-            uint32_t pt;
-            if constexpr (COL) pt = ktid >> (p + 1);
-            else pt = ((o << LOG_G) | ktid) >> (p + 1);
+            // thread part of the entry offset lp; the register part is added per pair
+            const uint32_t pt = ktid >> (p + 1);
             const bool fin = !FWD && final_inv && (p == LOG_G - 1);
-            const uint32_t tbase = FWD ? (n >> (gb + 1)) + pt : n + 1u - (n >> gb) + pt;
             uint64_t w[E], wq[E];
             static_for<0, E>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
@@ -222,7 +261,7 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
                     constexpr uint32_t pr = Gm::kr(c, r) >> (p + 1);
                     if constexpr (rp == r) {
                         if (!fin) {
-                            const ulonglong2 tv = ld_g2(tw + tbase + pr);
+                            const ulonglong2 tv = tws.template get<p>(pt + pr);
                             w[r] = tv.x;
                             wq[r] = tv.y;
                         }
@@ -303,6 +342,15 @@ ntt_pass_kernel(PassArgs a) {
 
     uint64_t v[E];
 
+    // ---- COL passes: stage the pass's twiddles in LDS (one contiguous range
+    // per pass, shared by every column of the limb); the barrier below the
+    // data loads publishes them.
+    ulonglong2 *tw_lds = reinterpret_cast<ulonglong2 *>(lds + L::WORDS);
+    if constexpr (COL) {
+        const uint32_t base = FWD ? 1u : n + 1u - (uint32_t)G;
+        for (uint32_t i = t; i < (uint32_t)G - 1u; i += NT) tw_lds[i] = ld_g2(tw + base + i);
+    }
+
     // ---- load: registers in chunk-0 layout
     constexpr bool LOAD_VIA_LDS = !COL && !FWD;  // inverse ROW: chunk 0 is thread-contiguous
     if constexpr (LOAD_VIA_LDS) {
@@ -353,7 +401,14 @@ ntt_pass_kernel(PassArgs a) {
     }
 
     // ---- chunks
-    run_chunks<LOG_G, LOG_E, GPW, FWD, COL>(v, lds, g, tid, o, blo, n, m, tw, a.final_inv != 0, J);
+    if constexpr (COL) {
+        __syncthreads();
+        const TwLds<LOG_G, FWD> tws{tw_lds};
+        run_chunks<LOG_G, LOG_E, GPW, FWD, COL>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
+    } else {
+        const TwGlobal<LOG_G, FWD, COL> tws{tw, n, blo, o};
+        run_chunks<LOG_G, LOG_E, GPW, FWD, COL>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
+    }
 
     // ---- normalisation folded into the last pass (ntt.cpp:79-118)
     if (a.normalize) {

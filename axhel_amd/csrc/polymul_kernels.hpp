@@ -89,13 +89,15 @@ polymul_row_kernel(PolyArgs a) {
         constexpr int r = decltype(rc)::value;
         va[r] = ld_g(P.a1 + (row | (k0 + Gf::kr(0, r))));
     });
-    run_chunks<LOG_G, LOG_E, GPW, true, false>(va, lds, g, tid, o, 0, n, m, P.twf, false, fin);
+    const TwGlobal<LOG_G, true, false> twf{P.twf, n, 0, o};
+    const TwGlobal<LOG_G, false, false> twi{P.twi, n, 0, o};
+    run_chunks<LOG_G, LOG_E, GPW, true, false>(va, lds, g, tid, m, twf, false, fin);
     static_for<0, E>([&](auto rc) {
         constexpr int r = decltype(rc)::value;
         vb[r] = ld_g(P.b1 + (row | (k0 + Gf::kr(0, r))));
     });
     __syncthreads();  // the LDS tile is reused by b's pass
-    run_chunks<LOG_G, LOG_E, GPW, true, false>(vb, lds, g, tid, o, 0, n, m, P.twf, false, fin);
+    run_chunks<LOG_G, LOG_E, GPW, true, false>(vb, lds, g, tid, m, twf, false, fin);
 
     // thread holds words row + kl + r, r in [0, E)
     const uint32_t kl = Gf::kt(Gf::NC - 1, tid);
@@ -128,7 +130,7 @@ polymul_row_kernel(PolyArgs a) {
         }
         va[r] = d;
     });
-    run_chunks<LOG_G, LOG_E, GPW, false, false>(va, lds, g, tid, o, 0, n, m, P.twi, a.single != 0, fin);
+    run_chunks<LOG_G, LOG_E, GPW, false, false>(va, lds, g, tid, m, twi, a.single != 0, fin);
     if (a.single) {
         static_for<0, E>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
