@@ -126,6 +126,7 @@ SIGNATURES = {
     "axhel_hip_ntt_batch_launch_kernel": (_st, [_p, ctypes.c_int, _p]),
     "axhel_hip_ntt_batch_run": (_st, [ctypes.POINTER(NttJob), ctypes.c_size_t, _p]),
     "axhel_hip_ntt_config_count": (ctypes.c_int, [_u64]),
+    "axhel_hip_ntt_batch_stacked_groups": (ctypes.c_int, [_p]),
     "axhel_hip_ntt_config_get": (_st, [_u64, ctypes.c_int, ctypes.POINTER(NttConfig)]),
     "axhel_hip_ntt_config_select": (_st, [_u64, ctypes.c_int]),
     "axhel_hip_ntt_config_selected": (ctypes.c_int, [_u64]),
@@ -356,6 +357,10 @@ class Batch:
 
     def launch_kernel(self, index, stream=None):
         _check(_load().axhel_hip_ntt_batch_launch_kernel(self._h, index, _stream_ptr(stream)), "launch_kernel")
+
+    def stacked_groups(self):
+        """Groups whose ROW pass runs over stacks of same-plan jobs."""
+        return int(_load().axhel_hip_ntt_batch_stacked_groups(self._h))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -29,6 +29,10 @@ struct axhel_hip_batch {
         uint32_t count;
         size_t offset;  // first descriptor in d_jobs
         bool vec16;     // all src/dst 16-byte aligned
+        // stacked ROW pass: stack table at d_jobs + stack_offset (stack_gpw 0: none)
+        size_t stack_offset;
+        uint32_t stacks;
+        int stack_gpw;
     };
     std::vector<Group> groups;
     NttJobDev *d_jobs = nullptr;
@@ -52,11 +56,27 @@ void destroy(axhel_hip_batch *b) {
     delete b;
 }
 
-hipError_t issue(axhel_hip_batch *b, hipStream_t stream) {
+StackTable stack_of(const axhel_hip_batch *b, const axhel_hip_batch::Group &g) {
+    StackTable t;
+    if (g.stack_gpw) {
+        t.jobs = b->d_jobs + g.stack_offset;
+        t.count = g.stacks;
+        t.gpw = g.stack_gpw;
+    }
+    return t;
+}
+
+hipError_t launch_group(const axhel_hip_batch *b, const axhel_hip_batch::Group &g, hipStream_t stream,
+                        int only_pass = -1) {
     static const NttJobDev none{};
+    const StackTable st = stack_of(b, g);
+    return launch_ntt(config_for(g.logn), g.logn, g.inverse, g.normalize, b->d_jobs + g.offset, none, g.count,
+                      stream, only_pass, g.vec16, &st);
+}
+
+hipError_t issue(axhel_hip_batch *b, hipStream_t stream) {
     for (const auto &g : b->groups) {
-        hipError_t e = launch_ntt(config_for(g.logn), g.logn, g.inverse, g.normalize, b->d_jobs + g.offset, none,
-                                  g.count, stream, -1, g.vec16);
+        hipError_t e = launch_group(b, g, stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -67,7 +87,6 @@ hipError_t issue(axhel_hip_batch *b, hipStream_t stream) {
 // side stream so independent groups become parallel graph branches, joined
 // back on the origin stream.
 hipError_t capture(axhel_hip_batch *b) {
-    static const NttJobDev none{};
     hipStream_t origin = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&origin, hipStreamNonBlocking);
     if (e != hipSuccess) return e;
@@ -86,16 +105,12 @@ hipError_t capture(axhel_hip_batch *b) {
     if (e == hipSuccess) {
         hipError_t ce = hipSuccess;
         if (nb == 0) {
-            const auto &g = b->groups[0];
-            ce = launch_ntt(config_for(g.logn), g.logn, g.inverse, g.normalize, b->d_jobs + g.offset, none, g.count,
-                            origin, -1, g.vec16);
+            ce = launch_group(b, b->groups[0], origin);
         } else {
             ce = hipEventRecord(fork_ev, origin);
             for (int i = 0; i < nb && ce == hipSuccess; ++i) ce = hipStreamWaitEvent(b->side[i], fork_ev, 0);
             for (size_t gi = 0; gi < b->groups.size() && ce == hipSuccess; ++gi) {
-                const auto &g = b->groups[gi];
-                ce = launch_ntt(config_for(g.logn), g.logn, g.inverse, g.normalize, b->d_jobs + g.offset, none,
-                                g.count, b->side[gi % nb], -1, g.vec16);
+                ce = launch_group(b, b->groups[gi], b->side[gi % nb]);
             }
             for (int i = 0; i < nb && ce == hipSuccess; ++i) {
                 ce = hipEventRecord(b->events[i], b->side[i]);
@@ -155,7 +170,23 @@ axhel_hip_status axhel_hip_ntt_batch_create(const axhel_hip_ntt_job *jobs, size_
             host.push_back(make_job(jobs[i].plan, jobs[i].operand, jobs[i].result, g.inverse));
             g.vec16 = g.vec16 && aligned16(jobs[i].operand) && aligned16(jobs[i].result);
         }
+        g.stack_offset = 0;
+        g.stacks = 0;
+        g.stack_gpw = 0;
         b->groups.push_back(g);
+    }
+    // Stack tables (after all plain descriptors) for groups whose ROW pass
+    // can run over same-plan stacks.
+    for (auto &g : b->groups) {
+        const int gpw = stack_gpw(g.logn);
+        std::vector<NttJobDev> st;
+        uint32_t stacks = 0;
+        if (gpw && build_stacks(host.data() + g.offset, g.count, gpw, st, stacks)) {
+            g.stack_offset = host.size();
+            g.stacks = stacks;
+            g.stack_gpw = gpw;
+            host.insert(host.end(), st.begin(), st.end());
+        }
     }
     // Largest groups first: they fill the device while smaller ones trail.
     std::stable_sort(b->groups.begin(), b->groups.end(),
@@ -164,8 +195,9 @@ axhel_hip_status axhel_hip_ntt_batch_create(const axhel_hip_ntt_job *jobs, size_
                      });
     hipError_t e = hipSuccess;
     if (count) {
-        e = hipMalloc(&b->d_jobs, count * sizeof(NttJobDev));
-        if (e == hipSuccess) e = hipMemcpy(b->d_jobs, host.data(), count * sizeof(NttJobDev), hipMemcpyHostToDevice);
+        e = hipMalloc(&b->d_jobs, host.size() * sizeof(NttJobDev));
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_jobs, host.data(), host.size() * sizeof(NttJobDev), hipMemcpyHostToDevice);
     }
     for (const auto &g : b->groups)
         if (e == hipSuccess) e = prepare_config(config_for(g.logn));
@@ -199,14 +231,19 @@ int axhel_hip_ntt_batch_kernel_count(const axhel_hip_batch *batch) {
     return k;
 }
 
+int axhel_hip_ntt_batch_stacked_groups(const axhel_hip_batch *batch) {
+    if (!batch) return 0;
+    int k = 0;
+    for (const auto &g : batch->groups) k += g.stack_gpw != 0 && g.stack_gpw == stack_gpw(g.logn);
+    return k;
+}
+
 axhel_hip_status axhel_hip_ntt_batch_launch_kernel(axhel_hip_batch *batch, int index, void *stream) {
-    static const NttJobDev none{};
     if (!batch || index < 0) return AXHEL_HIP_INVALID_ARGUMENT;
     for (const auto &g : batch->groups) {
         const NttConfig &cfg = config_for(g.logn);
         if (index < cfg.npass) {
-            AXHEL_HIP_TRY(launch_ntt(cfg, g.logn, g.inverse, g.normalize, batch->d_jobs + g.offset, none, g.count,
-                                     as_stream(stream), index, g.vec16));
+            AXHEL_HIP_TRY(launch_group(batch, g, as_stream(stream), index));
             return AXHEL_HIP_OK;
         }
         index -= cfg.npass;

@@ -101,6 +101,21 @@ PassKernel pk_fused() {
     return k;
 }
 
+template <int LG, int LE, int GPW, bool FWD>
+PassKernel pk_stack() {
+    PassKernel k;
+    k.fn = reinterpret_cast<const void *>(&axhel_dev::ntt_row_stack_kernel<LG, LE, GPW, FWD>);
+    k.log_g = LG;
+    k.log_e = LE;
+    k.gpw = GPW;
+    k.col = false;
+    k.multi = false;
+    k.stack = true;
+    k.threads = GPW << (LG - LE);
+    k.lds_bytes = ((size_t)axhel_dev::Lds<LG, LE, GPW, false>::WORDS + 2 * ((1u << LG) - 1)) * sizeof(uint64_t);
+    return k;
+}
+
 template <int LG, int LE, int GPW>
 NttConfig single() {
     NttConfig c;
@@ -131,6 +146,8 @@ NttConfig two() {
     c.inv[0] = pk<LR, LER, GR, false, false, false>();
     c.inv[1] = pk<LC, LEC, GC, true, false, false>();
     c.fused = pk_fused<LR, LER, GR>();
+    c.fwd_stack = pk_stack<LR, LER, GR, true>();
+    c.inv_stack = pk_stack<LR, LER, GR, false>();
     return c;
 }
 
@@ -245,8 +262,46 @@ hipError_t prepare_config(const NttConfig &cfg) {
         if (e == hipSuccess) e = ensure_lds(cfg.inv[p]);
         if (e != hipSuccess) return e;
     }
+    if (cfg.fwd_stack.fn) {
+        hipError_t e = ensure_lds(cfg.fwd_stack);
+        if (e == hipSuccess) e = ensure_lds(cfg.inv_stack);
+        if (e != hipSuccess) return e;
+    }
     if (cfg.fused.fn) return ensure_lds(cfg.fused);
     return hipSuccess;
+}
+
+int stack_gpw(uint32_t logn) {
+    if (configs_for(logn).empty()) return 0;
+    const NttConfig &c = config_for(logn);
+    return c.fwd_stack.fn ? c.fwd_stack.gpw : 0;
+}
+
+bool build_stacks(const NttJobDev *jobs, uint32_t count, int gpw, std::vector<NttJobDev> &out, uint32_t &stacks) {
+    out.clear();
+    stacks = 0;
+    if (gpw <= 0 || count == 0) return false;
+    uint64_t slots = 0;
+    for (uint32_t i = 0; i < count;) {
+        uint32_t e = i + 1;
+        while (e < count && jobs[e].tw == jobs[i].tw && jobs[e].q == jobs[i].q) ++e;
+        slots += ((uint64_t)(e - i) + gpw - 1) / gpw * gpw;
+        i = e;
+    }
+    if ((slots - count) * 8 > slots) return false;
+    out.reserve(slots);
+    for (uint32_t i = 0; i < count;) {
+        uint32_t e = i + 1;
+        while (e < count && jobs[e].tw == jobs[i].tw && jobs[e].q == jobs[i].q) ++e;
+        for (uint32_t k = i; k < e; ++k) out.push_back(jobs[k]);
+        NttJobDev empty = jobs[i];
+        empty.src = nullptr;
+        empty.dst = nullptr;
+        while (out.size() % gpw) out.push_back(empty);
+        i = e;
+    }
+    stacks = (uint32_t)(out.size() / gpw);
+    return true;
 }
 
 NttJobDev make_job(const axhel_hip_plan *p, const uint64_t *src, uint64_t *dst, bool inverse) {
@@ -263,12 +318,16 @@ NttJobDev make_job(const axhel_hip_plan *p, const uint64_t *src, uint64_t *dst, 
 }
 
 hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool normalize, const NttJobDev *jobs,
-                      const NttJobDev &inline_job, uint32_t count, hipStream_t stream, int only_pass, bool vec16) {
+                      const NttJobDev &inline_job, uint32_t count, hipStream_t stream, int only_pass, bool vec16,
+                      const StackTable *stacks) {
     if (count == 0) return hipSuccess;
     const uint64_t n = 1ull << logn;
     for (int p = 0; p < cfg.npass; ++p) {
         if (only_pass >= 0 && p != only_pass) continue;
-        const PassKernel &k = inverse ? cfg.inv[p] : cfg.fwd[p];
+        const bool row_pass = cfg.npass == 2 && p == (inverse ? 0 : 1);
+        const PassKernel &ks = inverse ? cfg.inv_stack : cfg.fwd_stack;
+        const bool stacked = row_pass && stacks && stacks->jobs && stacks->count && ks.fn && ks.gpw == stacks->gpw;
+        const PassKernel &k = stacked ? ks : (inverse ? cfg.inv[p] : cfg.fwd[p]);
         hipError_t e = ensure_lds(k);
         if (e != hipSuccess) return e;
         PassArgs a;
@@ -283,8 +342,15 @@ hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool no
         a.final_inv = inverse && (p == cfg.npass - 1);
         a.vec16 = vec16 ? 1u : 0u;
         uint64_t blocks;
-        if (k.multi) blocks = (count + k.gpw - 1) / k.gpw;
-        else blocks = (uint64_t)count * a.groups_per_limb / k.gpw;
+        if (stacked) {
+            a.jobs = stacks->jobs;
+            a.num_limbs = stacks->count;
+            blocks = (uint64_t)stacks->count * a.groups_per_limb;
+        } else if (k.multi) {
+            blocks = (count + k.gpw - 1) / k.gpw;
+        } else {
+            blocks = (uint64_t)count * a.groups_per_limb / k.gpw;
+        }
         void *args[] = {&a};
         e = hipLaunchKernel(k.fn, dim3((unsigned)blocks), dim3(k.threads), args, k.lds_bytes, stream);
         if (e != hipSuccess) return e;

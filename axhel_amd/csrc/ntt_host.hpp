@@ -34,6 +34,7 @@ struct PassKernel {
     int gpw = 0;
     bool col = false;
     bool multi = false;
+    bool stack = false;  // ROW pass over stacks of same-plan jobs
     int threads = 0;
     size_t lds_bytes = 0;
 };
@@ -47,7 +48,24 @@ struct NttConfig {
     PassKernel fwd[2];
     PassKernel inv[2];
     PassKernel fused;
+    // Two-pass configurations: the ROW pass over stacks of jobs sharing one
+    // plan (forward pass 1, inverse pass 0), used when a stack table is given.
+    PassKernel fwd_stack, inv_stack;
 };
+
+// Jobs of one launch regrouped for the stacked ROW pass: `count` stacks of
+// `gpw` slots, each stack's jobs sharing a plan, empty slots with dst null.
+struct StackTable {
+    const axhel_dev::NttJobDev *jobs = nullptr;
+    uint32_t count = 0;
+    int gpw = 0;
+};
+
+// Builds the stacked job table from `jobs` ordered so that jobs of one plan
+// are adjacent (runs of equal table and modulus).  Returns false, leaving
+// `out` empty, when empty slots would exceed one in eight.
+bool build_stacks(const axhel_dev::NttJobDev *jobs, uint32_t count, int gpw, std::vector<axhel_dev::NttJobDev> &out,
+                  uint32_t &stacks);
 
 // Compiled configurations for log2(degree) (empty for unsupported degrees).
 const std::vector<NttConfig> &configs_for(uint32_t logn);
@@ -65,9 +83,16 @@ axhel_dev::NttJobDev make_job(const axhel_hip_plan *p, const uint64_t *src, uint
 // degree and direction whose descriptors are `jobs` (device table) or, when
 // jobs == nullptr and count == 1, `inline_job`.  only_pass >= 0 launches
 // that pass alone (per-kernel timing).  vec16: every src/dst is 16-byte
-// aligned (enables 16-byte global accesses).
+// aligned (enables 16-byte global accesses).  stacks: the same jobs as a
+// stack table; the ROW pass of a two-pass configuration then runs stacked
+// when the table was built for that pass's group count.
 hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool normalize,
                       const axhel_dev::NttJobDev *jobs, const axhel_dev::NttJobDev &inline_job, uint32_t count,
-                      hipStream_t stream, int only_pass = -1, bool vec16 = false);
+                      hipStream_t stream, int only_pass = -1, bool vec16 = false,
+                      const StackTable *stacks = nullptr);
+
+// Group count per stack of the stacked ROW pass for log2(degree) under the
+// selected configuration (0: no stacked pass).
+int stack_gpw(uint32_t logn);
 
 }  // namespace axhel_host

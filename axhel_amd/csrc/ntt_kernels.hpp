@@ -468,4 +468,144 @@ ntt_pass_kernel(PassArgs a) {
     }
 }
 
+// ROW pass over stacks of jobs that share one plan (same modulus and
+// twiddle table).  Workgroup (stack s, row o) transforms row o of the GPW
+// jobs jobs[s*GPW .. s*GPW+GPW-1]; an empty slot has dst == nullptr, slot 0
+// is always a job.  Every row of the workgroup uses the same 2^LOG_G - 1
+// twiddle pairs, which are staged in LDS once (for the per-limb ROW pass
+// each row needs its own set, twice the row's own bytes, from L2).  Bits,
+// butterflies and twiddles are those of ntt_pass_kernel's ROW pass.
+template <int LOG_G, int LOG_E, int GPW, bool FWD>
+__global__ void __launch_bounds__(GPW << (LOG_G - LOG_E))
+ntt_row_stack_kernel(PassArgs a) {
+    using Gm = Geom<LOG_G, LOG_E, FWD>;
+    using L = Lds<LOG_G, LOG_E, GPW, false>;
+    constexpr int G = 1 << LOG_G;
+    constexpr int E = 1 << LOG_E;
+    constexpr int T = G / E;
+    constexpr int NT = GPW * T;
+    constexpr int NC = Gm::NC;
+    extern __shared__ uint64_t lds[];
+
+    uint32_t tile = blockIdx.x;
+    {
+        const uint32_t nb = gridDim.x;
+        if ((nb & 7u) == 0) tile = (tile & 7u) * (nb >> 3) + (tile >> 3);
+    }
+    const uint32_t t = threadIdx.x;
+    const uint32_t tid = t % T, g = t / T;
+    const uint32_t st = tile / a.groups_per_limb;     // stack (num_limbs = number of stacks)
+    const uint32_t o = tile - st * a.groups_per_limb;  // row
+    const NttJobDev J0 = a.jobs[st * GPW];              // the stack's plan
+    const NttJobDev *Jg = a.jobs + st * GPW + g;
+    uint64_t *dst = Jg->dst;
+    const uint64_t *src = a.first_pass ? Jg->src : dst;
+    const bool valid = dst != nullptr;
+    const ModConsts m = mod_consts(J0.q);
+    const uint32_t n = 1u << a.logn;
+
+    // Row o's twiddles into LDS, in the TwLds order: entry i is (j, lp) of
+    // tw_lds_index, read from the table at the stage base plus (o << j) + lp.
+    ulonglong2 *tw_lds = reinterpret_cast<ulonglong2 *>(lds + L::WORDS);
+    for (uint32_t i = t; i < (uint32_t)G - 1u; i += NT) {
+        uint32_t j, lp, base;
+        if constexpr (FWD) {
+            j = 31u - __clz(i + 1u);
+            lp = i + 1u - (1u << j);
+            base = n >> (LOG_G - j);
+        } else {
+            j = 31u - __clz((uint32_t)G - i - 1u);
+            lp = i - ((uint32_t)G - (2u << j));
+            base = n + 1u - (n >> (LOG_G - 1 - j));
+        }
+        tw_lds[i] = ld_g2(J0.tw + base + (o << j) + lp);
+    }
+
+    const uint64_t row = (uint64_t)o << LOG_G;
+    uint64_t v[E];
+    if constexpr (!FWD) {
+        // inverse: chunk 0 is thread-contiguous; copy the thread's own row
+        // through LDS in 16-byte units (T threads x E/2 units per row)
+        if (valid) {
+            if (a.vec16) {
+#pragma unroll
+                for (int i = 0; i < E / 2; ++i) {
+                    const uint32_t k = 2u * (tid + (uint32_t)i * T);
+                    const ulonglong2 x2 = ld_nt2(src + row + k);
+                    lds[L::idx(g, k)] = x2.x;
+                    lds[L::idx(g, k + 1)] = x2.y;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < E; ++i) {
+                    const uint32_t k = tid + (uint32_t)i * T;
+                    lds[L::idx(g, k)] = ld_g(src + row + k);
+                }
+            }
+        }
+        __syncthreads();  // also publishes the twiddles
+        const uint32_t k0 = Gm::kt(0, tid);
+        static_for<0, E>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            v[r] = lds[L::idx(g, k0 + Gm::kr(0, r))];
+        });
+    } else {
+        const uint32_t k0 = Gm::kt(0, tid);
+        if (valid) {
+            static_for<0, E>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                v[r] = ld_g(src + row + k0 + Gm::kr(0, r));
+            });
+        } else {
+            static_for<0, E>([&](auto rc) { v[decltype(rc)::value] = 0; });
+        }
+        __syncthreads();  // publishes the twiddles
+    }
+
+    const TwLds<LOG_G, FWD> tws{tw_lds};
+    run_chunks<LOG_G, LOG_E, GPW, FWD, false>(v, lds, g, tid, m, tws, false, J0);
+
+    if (a.normalize) {
+        static_for<0, E>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            if constexpr (FWD) v[r] = csub_n(csub_n(v[r], m.q2, m.nq2), m.q, m.nq);
+            else v[r] = csub_n(v[r], m.q, m.nq);
+        });
+    }
+
+    const uint32_t kl = Gm::kt(NC - 1, tid);
+    if constexpr (FWD) {
+        // forward: the last chunk is thread-contiguous; store through LDS
+        __syncthreads();
+        static_for<0, E>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            lds[L::idx(g, kl + Gm::kr(NC - 1, r))] = v[r];
+        });
+        __syncthreads();
+        if (valid) {
+            if (a.vec16) {
+#pragma unroll
+                for (int i = 0; i < E / 2; ++i) {
+                    const uint32_t k = 2u * (tid + (uint32_t)i * T);
+                    ulonglong2 x2;
+                    x2.x = lds[L::idx(g, k)];
+                    x2.y = lds[L::idx(g, k + 1)];
+                    st_g2(dst + row + k, x2);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < E; ++i) {
+                    const uint32_t k = tid + (uint32_t)i * T;
+                    st_g(dst + row + k, lds[L::idx(g, k)]);
+                }
+            }
+        }
+    } else if (valid) {
+        static_for<0, E>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            st_g(dst + row + kl + Gm::kr(NC - 1, r), v[r]);
+        });
+    }
+}
+
 }  // namespace axhel_dev

@@ -151,6 +151,12 @@ axhel_hip_status axhel_hip_ntt_batch_destroy(axhel_hip_batch *batch);
    alone on `stream`, in replay order (per-kernel timing with events). */
 int axhel_hip_ntt_batch_kernel_count(const axhel_hip_batch *batch);
 axhel_hip_status axhel_hip_ntt_batch_launch_kernel(axhel_hip_batch *batch, int index, void *stream);
+/* Number of the batch's (degree, direction) groups whose ROW pass runs over
+   stacks of jobs sharing one plan (one twiddle set per workgroup).  A group
+   is stacked when its configuration has two passes and padding each plan's
+   jobs to whole stacks leaves at most one slot in eight empty.  Diagnostic;
+   no reference counterpart. */
+int axhel_hip_ntt_batch_stacked_groups(const axhel_hip_batch *batch);
 /* One-shot batch: create, launch eagerly on `stream`, synchronize, destroy. */
 axhel_hip_status axhel_hip_ntt_batch_run(const axhel_hip_ntt_job *jobs, size_t count, void *stream);
 

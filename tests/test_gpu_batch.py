@@ -106,3 +106,67 @@ def test_in_place_batch_jobs():
     b.launch()
     for d, want in refs:
         assert np.array_equal(host(d), want)
+
+
+@pytest.mark.parametrize("logn", [14, 15, 16, 17])
+def test_stacked_row_pass_every_config(logn):
+    """Many jobs per plan: the ROW pass runs over stacks of same-plan jobs
+    (one twiddle set per workgroup, a padded last stack per plan).  For every
+    two-pass configuration, both directions and both output ranges, every
+    job equals the plain per-limb call and one job per plan equals the
+    oracle."""
+    torch = torch_mod()
+    n = 1 << logn
+    qs = params.rns_primes(n, 2)
+    psis = [params.find_primitive_2n_root(n, q) for q in qs]
+    plans = [ax.NTT(n, q, p) for q, p in zip(qs, psis)]
+    tabs = [O.Tables(n, q, p) for q, p in zip(qs, psis)]
+    rng = np.random.default_rng(logn)
+    try:
+        for ci in range(ax.config_count(n)):
+            ax.config_select(n, ci)
+            gpw = ax.config_get(n, ci)["row_groups"]
+            counts = [4 * gpw, 3 * gpw + 1]  # one padded stack: 1 empty slot in 8 or fewer
+            owner = [pi for pi, c in enumerate(counts) for _ in range(c)]
+            for inverse in (False, True):
+                f_in = 2 if inverse else 4
+                host_in = np.stack([params.uniform(rng, n, f_in * qs[pi]) for pi in owner])
+                src = dev(host_in.reshape(-1)).view(len(owner), n)
+                for lazy in (True, False):
+                    of = f_in if lazy else 1
+                    out = torch.full_like(src, -1)
+                    b = ax.Batch([(plans[pi], out[i], src[i], inverse, of) for i, pi in enumerate(owner)])
+                    assert b.stacked_groups() == 1, (logn, ci)
+                    b.launch()
+                    ref = torch.full_like(src, -1)
+                    for i, pi in enumerate(owner):
+                        call = plans[pi].ComputeInverse if inverse else plans[pi].ComputeForward
+                        call(ref[i], src[i], f_in, of)
+                    torch.cuda.synchronize()
+                    assert torch.equal(out, ref), (logn, ci, inverse, lazy)
+                    got = host(out).reshape(len(owner), n)
+                    for pi in range(len(counts)):
+                        i = owner.index(pi) + counts[pi] - 1  # last job of the plan (in the padded stack)
+                        want = (O.compute_inverse(host_in[i], tabs[pi], f_in, of) if inverse
+                                else O.compute_forward(host_in[i], tabs[pi], f_in, of))
+                        assert np.array_equal(got[i], want), (logn, ci, inverse, lazy, pi)
+                    b.close()
+    finally:
+        ax.config_select(n, -1)
+
+
+def test_sparse_plans_do_not_stack():
+    """One job per plan would leave most stack slots empty: the batch keeps
+    the per-limb ROW pass."""
+    torch = torch_mod()
+    n = 1 << 15
+    qs = params.rns_primes(n, 6)
+    plans = [ax.NTT(n, q, params.find_primitive_2n_root(n, q)) for q in qs]
+    src = torch.zeros((6, n), dtype=torch.int64, device="cuda")
+    out = torch.empty_like(src)
+    b = ax.Batch([(plans[i], out[i], src[i], False, 1) for i in range(6)])
+    assert b.stacked_groups() == 0
+    b.launch()
+    torch.cuda.synchronize()
+    assert int(out.abs().sum()) == 0
+    b.close()
