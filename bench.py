@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+VALU_ISSUE_PEAK = 1024 * 2.4 / 4  # G wave64 VALU instructions/s: 256 CUs x 4 SIMDs, 2.4 GHz, 4 cycles each
 
 
 def log(*a):
@@ -204,7 +205,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--polys", type=int, default=64)
     ap.add_argument("--limbs", type=int, default=24)
     ap.add_argument("--logn", type=int, default=16)
@@ -256,33 +257,35 @@ def main():
     batch = ax.Batch(jobs)
     limbs = P * L
 
-    # the timed steps run on a dedicated stream (graph replays on the NULL
-    # stream pick up its implicit synchronisation)
+    # Timed steps: graph replays on the default stream, as Batch.launch()
+    # issues them.  Beside it: the same replays on a separate stream, and the
+    # same kernels as plain launches (no graph) on that stream.
     run_stream = torch.cuda.Stream()
     for _ in range(args.warmup):
+        batch.launch()
         batch.launch(run_stream)
         batch.launch_eager(run_stream)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        batch.launch(run_stream)
+        batch.launch()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms = el * 1e3 / args.steps
     value = limbs * args.steps / el
-    log(f"batched forward (graph replay): {ms:.3f} ms/step, {value:.4g} limbs/s")
+    log(f"batched forward (graph replay, default stream): {ms:.3f} ms/step, {value:.4g} limbs/s")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.launch(run_stream)
+    torch.cuda.synchronize()
+    stream_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log(f"graph replay on a separate stream: {stream_ms:.3f} ms/step")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         batch.launch_eager(run_stream)
     torch.cuda.synchronize()
     eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     log(f"same step as eager launches: {eager_ms:.3f} ms/step")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.launch()
-    torch.cuda.synchronize()
-    null_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    log(f"graph replay on the NULL stream: {null_ms:.3f} ms/step")
 
     # inverse (reported beside)
     ibatch = ax.Batch([(plans[l], src[p, l], dst[p, l], True, 1) for p in range(P) for l in range(L)])
@@ -332,7 +335,22 @@ def main():
             if line.startswith("traffic_bytes"):
                 kv = dict(x.split("=") for x in line.split()[1:])
                 traffic = float(kv.get(f"pass{dom}", "nan"))
-    log(f"dominant pass {dom}: {achieved_bfly:.1f} Gbfly/s of {peak_bfly:.1f} peak; {achieved_gbs:.0f} GB/s")
+    # VALU issue: wave64 VALU instructions per launch (SQ_INSTS_VALU of the
+    # committed counter profile of this workload) over the launch duration,
+    # against gfx950's issue peak: one wave64 VALU instruction per 4 cycles
+    # per SIMD, 1024 SIMDs, 2.4 GHz.
+    valu_issue = None
+    vpath = os.path.join(ROOT, "profiles", "ntt_n65536_64x24_valu_counters.txt")
+    if os.path.exists(vpath) and args.logn == 16 and P == 64 and L == 24:
+        for line in open(vpath):
+            if line.startswith("valu_insts"):
+                kv = dict(x.split("=") for x in line.split()[1:])
+                insts = float(kv.get(f"pass{dom}", "nan"))
+                ach = insts / (kdur[dom] * 1e-3) / 1e9
+                valu_issue = {"achieved": ach, "peak": VALU_ISSUE_PEAK, "unit": "G wave64 VALU instr/s",
+                              "frac": ach / VALU_ISSUE_PEAK, "insts_per_launch": insts}
+    log(f"dominant pass {dom}: {achieved_bfly:.1f} Gbfly/s of {peak_bfly:.1f} peak; {achieved_gbs:.0f} GB/s"
+        + (f"; VALU issue {valu_issue['frac']:.2f} of peak" if valu_issue else ""))
 
     # plain per-call path: one synchronous ComputeForward per limb
     pl = min(args.plain_limbs, limbs)
@@ -386,20 +404,21 @@ def main():
         "dtype": "u64",
         "data": "synthetic: uniform residues in [0,q) per limb, generated on device (seed 0)",
         "config": {"workload": f"forward negacyclic NTT, {P} polys x {L} limbs x N={n}, "
-                               "24 primes 50-60 bits, normalized output, one hipGraph replay per step",
+                               "24 primes 50-60 bits, normalized output, one hipGraph replay per step on the default stream",
                    "polys": P, "limbs": L, "degree": n, "kernel_config": cfg},
         "plain_limbs_per_s": plain,
         "batched_vs_plain": value / plain,
         "inverse_limbs_per_s": inv_rate,
         "kernel_ms": kdur,
         "eager_ms_per_step": eager_ms,
-        "null_stream_graph_ms_per_step": null_ms,
+        "separate_stream_graph_ms_per_step": stream_ms,
         "roofline": {"bound": "valu", "achieved": achieved_bfly, "peak": peak_bfly, "unit": "Gbfly/s",
                      "frac": achieved_bfly / peak_bfly, "traffic": traffic,
                      "kernel": f"ntt pass {dom} of {kcount} ({kdur[dom]:.4f} ms)",
                      "peak_source": "register-resident Harvey butterflies, all CUs (axhel_hip_ubench_butterfly_peak)",
                      "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": achieved_gbs / HBM_PEAK_GBS}},
+                             "frac": achieved_gbs / HBM_PEAK_GBS},
+                     "valu_issue": valu_issue},
         "eltwise": elt,
         "polymul": pmul,
         "uneven_batch": uneven,
