@@ -134,6 +134,7 @@ SIGNATURES = {
                                        ctypes.POINTER(_p)]),
     "axhel_hip_polymul_launch": (_st, [_p, _p]),
     "axhel_hip_polymul_destroy": (_st, [_p]),
+    "axhel_hip_polymul_stacked_groups": (ctypes.c_int, [_p]),
     "axhel_hip_server_create": (_st, [ctypes.POINTER(ServeConfig), ctypes.POINTER(_p)]),
     "axhel_hip_server_destroy": (_st, [_p]),
     "axhel_hip_server_plan": (_st, [_p, _u64, _u64, _u64, ctypes.POINTER(_p)]),
@@ -406,6 +407,10 @@ class Polymul:
 
     def launch(self, stream=None):
         _check(_load().axhel_hip_polymul_launch(self._h, _stream_ptr(stream)), "polymul_launch")
+
+    def stacked_groups(self):
+        """Degree groups whose fused kernel runs over stacks of same-plan jobs."""
+        return int(_load().axhel_hip_polymul_stacked_groups(self._h))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

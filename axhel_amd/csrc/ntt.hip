@@ -101,6 +101,21 @@ PassKernel pk_fused() {
     return k;
 }
 
+template <int LG, int LE, int GPW>
+PassKernel pk_fused_stack() {
+    PassKernel k;
+    k.fn = reinterpret_cast<const void *>(&axhel_dev::polymul_row_stack_kernel<LG, LE, GPW>);
+    k.log_g = LG;
+    k.log_e = LE;
+    k.gpw = GPW;
+    k.col = false;
+    k.multi = false;
+    k.stack = true;
+    k.threads = GPW << (LG - LE);
+    k.lds_bytes = ((size_t)axhel_dev::Lds<LG, LE, GPW, false>::WORDS + 4 * ((1u << LG) - 1)) * sizeof(uint64_t);
+    return k;
+}
+
 template <int LG, int LE, int GPW, bool FWD>
 PassKernel pk_stack() {
     PassKernel k;
@@ -148,6 +163,7 @@ NttConfig two() {
     c.fused = pk_fused<LR, LER, GR>();
     c.fwd_stack = pk_stack<LR, LER, GR, true>();
     c.inv_stack = pk_stack<LR, LER, GR, false>();
+    c.fused_stack = pk_fused_stack<LR, LER, GR>();
     return c;
 }
 
@@ -265,6 +281,7 @@ hipError_t prepare_config(const NttConfig &cfg) {
     if (cfg.fwd_stack.fn) {
         hipError_t e = ensure_lds(cfg.fwd_stack);
         if (e == hipSuccess) e = ensure_lds(cfg.inv_stack);
+        if (e == hipSuccess && cfg.fused_stack.fn) e = ensure_lds(cfg.fused_stack);
         if (e != hipSuccess) return e;
     }
     if (cfg.fused.fn) return ensure_lds(cfg.fused);

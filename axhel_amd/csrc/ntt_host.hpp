@@ -51,6 +51,8 @@ struct NttConfig {
     // Two-pass configurations: the ROW pass over stacks of jobs sharing one
     // plan (forward pass 1, inverse pass 0), used when a stack table is given.
     PassKernel fwd_stack, inv_stack;
+    // Fused polynomial-product kernel over the same stacks (fn null: none).
+    PassKernel fused_stack;
 };
 
 // Jobs of one launch regrouped for the stacked ROW pass: `count` stacks of

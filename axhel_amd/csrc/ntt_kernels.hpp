@@ -468,6 +468,29 @@ ntt_pass_kernel(PassArgs a) {
     }
 }
 
+// Copies the 2^LOG_G - 1 twiddle pairs of ROW-pass row o (degree n) into
+// LDS in the TwLds order: entry i is (j, lp) of tw_lds_index, read from the
+// table at the stage base plus (o << j) + lp.  Cooperative over nthreads
+// threads; the caller's barrier publishes the copy.
+template <int LOG_G, bool FWD>
+__device__ __forceinline__ void stage_row_twiddles(ulonglong2 *tw_lds, const ulonglong2 *tw, uint32_t n, uint32_t o,
+                                                   uint32_t t, uint32_t nthreads) {
+    constexpr uint32_t G = 1u << LOG_G;
+    for (uint32_t i = t; i < G - 1u; i += nthreads) {
+        uint32_t j, lp, base;
+        if constexpr (FWD) {
+            j = 31u - __clz(i + 1u);
+            lp = i + 1u - (1u << j);
+            base = n >> (LOG_G - j);
+        } else {
+            j = 31u - __clz(G - i - 1u);
+            lp = i - (G - (2u << j));
+            base = n + 1u - (n >> (LOG_G - 1 - j));
+        }
+        tw_lds[i] = ld_g2(tw + base + (o << j) + lp);
+    }
+}
+
 // ROW pass over stacks of jobs that share one plan (same modulus and
 // twiddle table).  Workgroup (stack s, row o) transforms row o of the GPW
 // jobs jobs[s*GPW .. s*GPW+GPW-1]; an empty slot has dst == nullptr, slot 0
@@ -504,22 +527,8 @@ ntt_row_stack_kernel(PassArgs a) {
     const ModConsts m = mod_consts(J0.q);
     const uint32_t n = 1u << a.logn;
 
-    // Row o's twiddles into LDS, in the TwLds order: entry i is (j, lp) of
-    // tw_lds_index, read from the table at the stage base plus (o << j) + lp.
     ulonglong2 *tw_lds = reinterpret_cast<ulonglong2 *>(lds + L::WORDS);
-    for (uint32_t i = t; i < (uint32_t)G - 1u; i += NT) {
-        uint32_t j, lp, base;
-        if constexpr (FWD) {
-            j = 31u - __clz(i + 1u);
-            lp = i + 1u - (1u << j);
-            base = n >> (LOG_G - j);
-        } else {
-            j = 31u - __clz((uint32_t)G - i - 1u);
-            lp = i - ((uint32_t)G - (2u << j));
-            base = n + 1u - (n >> (LOG_G - 1 - j));
-        }
-        tw_lds[i] = ld_g2(J0.tw + base + (o << j) + lp);
-    }
+    stage_row_twiddles<LOG_G, FWD>(tw_lds, J0.tw, n, o, t, NT);
 
     const uint64_t row = (uint64_t)o << LOG_G;
     uint64_t v[E];

@@ -30,6 +30,11 @@ struct axhel_hip_polymul {
         size_t off_elt_mul, off_elt_fma;  // count EltLimbDev each, unfused path
         bool vec16;
         bool has_addend;
+        // fused path over stacks of same-plan jobs: stack table at off_fstack
+        // (fstack_gpw 0: none)
+        size_t off_fstack;
+        uint32_t fstacks;
+        int fstack_gpw;
     };
     std::vector<Group> groups;
     char *d_desc = nullptr;
@@ -44,6 +49,38 @@ size_t put(std::vector<char> &buf, const std::vector<T> &v) {
     buf.resize(off + v.size() * sizeof(T));
     std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
     return off;
+}
+
+// Stacks of gpw same-plan product jobs (runs of equal modulus and tables in
+// `jobs`, which are ordered by plan), padded with empty slots (out null);
+// false when padding would leave more than one slot in eight empty.
+bool stack_poly_jobs(const std::vector<PolyJobDev> &jobs, int gpw, std::vector<PolyJobDev> &out, uint32_t &stacks) {
+    out.clear();
+    stacks = 0;
+    const size_t count = jobs.size();
+    if (gpw <= 0 || count == 0) return false;
+    auto same = [&](size_t x, size_t y) {
+        return jobs[x].twf == jobs[y].twf && jobs[x].twi == jobs[y].twi && jobs[x].q == jobs[y].q;
+    };
+    uint64_t slots = 0;
+    for (size_t i = 0; i < count;) {
+        size_t e = i + 1;
+        while (e < count && same(e, i)) ++e;
+        slots += (e - i + gpw - 1) / gpw * gpw;
+        i = e;
+    }
+    if ((slots - count) * 8 > slots) return false;
+    for (size_t i = 0; i < count;) {
+        size_t e = i + 1;
+        while (e < count && same(e, i)) ++e;
+        for (size_t k = i; k < e; ++k) out.push_back(jobs[k]);
+        PolyJobDev empty = jobs[i];
+        empty.out = nullptr;
+        while (out.size() % gpw) out.push_back(empty);
+        i = e;
+    }
+    stacks = (uint32_t)(out.size() / gpw);
+    return true;
 }
 
 hipError_t issue(axhel_hip_polymul *pm, hipStream_t st) {
@@ -67,10 +104,16 @@ hipError_t issue(axhel_hip_polymul *pm, hipStream_t st) {
             a.groups_per_limb = (uint32_t)((1ull << g.logn) >> cfg.fused.log_g);
             a.single = cfg.npass == 1;
             a.vec16 = g.vec16;
-            const uint64_t blocks = (uint64_t)g.count * a.groups_per_limb / cfg.fused.gpw;
+            const bool stacked = g.fstack_gpw && cfg.fused_stack.fn && cfg.fused_stack.gpw == g.fstack_gpw;
+            const PassKernel &fk = stacked ? cfg.fused_stack : cfg.fused;
+            uint64_t blocks = (uint64_t)g.count * a.groups_per_limb / cfg.fused.gpw;
+            if (stacked) {
+                a.jobs = (const PolyJobDev *)(pm->d_desc + g.off_fstack);
+                a.num_limbs = g.fstacks;
+                blocks = (uint64_t)g.fstacks * a.groups_per_limb;
+            }
             void *args[] = {&a};
-            if ((e = hipLaunchKernel(cfg.fused.fn, dim3((unsigned)blocks), dim3(cfg.fused.threads), args,
-                                     cfg.fused.lds_bytes, st)))
+            if ((e = hipLaunchKernel(fk.fn, dim3((unsigned)blocks), dim3(fk.threads), args, fk.lds_bytes, st)))
                 return e;
             if (cfg.npass == 2) {
                 if ((e = launch_ntt(cfg, g.logn, true, true, inv, none, g.count, st, 1, g.vec16))) return e;
@@ -171,6 +214,15 @@ axhel_hip_status axhel_hip_polymul_create(const axhel_hip_polymul_job *jobs, siz
         g.off_fused = put(buf, fz);
         g.off_elt_mul = put(buf, mul);
         g.off_elt_fma = put(buf, fma);
+        g.off_fstack = 0;
+        g.fstacks = 0;
+        g.fstack_gpw = 0;
+        const NttConfig &cfg = config_for(g.logn);
+        std::vector<PolyJobDev> fst;
+        if (cfg.npass == 2 && cfg.fused_stack.fn && stack_poly_jobs(fz, cfg.fused_stack.gpw, fst, g.fstacks)) {
+            g.off_fstack = put(buf, fst);
+            g.fstack_gpw = cfg.fused_stack.gpw;
+        }
         pm->groups.push_back(g);
     }
     hipError_t e = hipSuccess;
@@ -192,6 +244,16 @@ axhel_hip_status axhel_hip_polymul_launch(axhel_hip_polymul *pm, void *stream) {
     if (!pm) return AXHEL_HIP_INVALID_ARGUMENT;
     AXHEL_HIP_TRY(issue(pm, as_stream(stream)));
     return AXHEL_HIP_OK;
+}
+
+int axhel_hip_polymul_stacked_groups(const axhel_hip_polymul *pm) {
+    if (!pm) return 0;
+    int k = 0;
+    for (const auto &g : pm->groups) {
+        const NttConfig &cfg = config_for(g.logn);
+        k += pm->fused && g.fstack_gpw != 0 && cfg.fused_stack.fn && cfg.fused_stack.gpw == g.fstack_gpw;
+    }
+    return k;
 }
 
 axhel_hip_status axhel_hip_polymul_destroy(axhel_hip_polymul *pm) {

@@ -144,4 +144,106 @@ polymul_row_kernel(PolyArgs a) {
     });
 }
 
+// The fused ROW kernel over stacks of jobs sharing one plan (two-pass
+// degrees): workgroup (stack s, row o) runs the fused steps on row o of the
+// GPW jobs jobs[s*GPW ..]; empty slots have out == nullptr, slot 0 is always
+// a job.  Row o's forward and inverse twiddles are staged in LDS once for all
+// GPW rows (each row needs its own 2 x (2^LOG_G - 1) pairs, which the
+// per-limb kernel reads from L2).  Same arithmetic as polymul_row_kernel.
+template <int LOG_G, int LOG_E, int GPW>
+__global__ void __launch_bounds__(GPW << (LOG_G - LOG_E))
+polymul_row_stack_kernel(PolyArgs a) {
+    static_assert(FusedLayout<LOG_G, LOG_E>::ok(), "forward exit layout must be the inverse entry layout");
+    using Gf = Geom<LOG_G, LOG_E, true>;
+    using Gi = Geom<LOG_G, LOG_E, false>;
+    using L = Lds<LOG_G, LOG_E, GPW, false>;
+    constexpr int G = 1 << LOG_G;
+    constexpr int E = 1 << LOG_E;
+    constexpr int T = G / E;
+    constexpr int NT = GPW * T;
+    extern __shared__ uint64_t lds[];
+
+    uint32_t tile = blockIdx.x;
+    {
+        const uint32_t nb = gridDim.x;
+        if ((nb & 7u) == 0) tile = (tile & 7u) * (nb >> 3) + (tile >> 3);
+    }
+    const uint32_t t = threadIdx.x;
+    const uint32_t tid = t % T, g = t / T;
+    const uint32_t st = tile / a.groups_per_limb;
+    const uint32_t o = tile - st * a.groups_per_limb;
+    const PolyJobDev P0 = a.jobs[st * GPW];  // the stack's plan
+    const PolyJobDev *Pg = a.jobs + st * GPW + g;
+    uint64_t *out = Pg->out;
+    const bool valid = out != nullptr;
+    const uint64_t *a1 = Pg->a1, *b1 = Pg->b1, *c = Pg->c;
+    const uint64_t scalar = Pg->scalar;
+    const ModConsts m = mod_consts(P0.q);
+    const uint32_t n = 1u << a.logn;
+    const uint64_t row = (uint64_t)o << LOG_G;
+
+    ulonglong2 *twf_lds = reinterpret_cast<ulonglong2 *>(lds + L::WORDS);
+    ulonglong2 *twi_lds = twf_lds + (G - 1);
+    stage_row_twiddles<LOG_G, true>(twf_lds, P0.twf, n, o, t, NT);
+    stage_row_twiddles<LOG_G, false>(twi_lds, P0.twi, n, o, t, NT);
+
+    NttJobDev fin{};  // no final stage here: it belongs to the inverse COL pass
+
+    uint64_t va[E], vb[E];
+    const uint32_t k0 = Gf::kt(0, tid);
+    static_for<0, E>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        va[r] = valid ? ld_g(a1 + row + k0 + Gf::kr(0, r)) : 0;
+    });
+    __syncthreads();  // publishes the twiddles
+    const TwLds<LOG_G, true> twf{twf_lds};
+    const TwLds<LOG_G, false> twi{twi_lds};
+    run_chunks<LOG_G, LOG_E, GPW, true, false>(va, lds, g, tid, m, twf, false, fin);
+    static_for<0, E>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        vb[r] = valid ? ld_g(b1 + row + k0 + Gf::kr(0, r)) : 0;
+    });
+    __syncthreads();  // the LDS tile is reused by b's pass
+    run_chunks<LOG_G, LOG_E, GPW, true, false>(vb, lds, g, tid, m, twf, false, fin);
+
+    const uint32_t kl = Gf::kt(Gf::NC - 1, tid);
+    uint64_t cv[E];
+    if (c && valid) {
+        if (a.vec16) {
+            static_for<0, E / 2>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const ulonglong2 x2 = ld_nt2(c + row + kl + 2 * i);
+                cv[2 * i] = x2.x;
+                cv[2 * i + 1] = x2.y;
+            });
+        } else {
+            static_for<0, E>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                cv[r] = ld_g(c + row + kl + r);
+            });
+        }
+    }
+    static_for<0, E>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        // NTT normalisation (ntt.cpp:79-91), then MulMod and FMAMod at factor 1
+        const uint64_t x = csub_n(csub_n(va[r], m.q2, m.nq2), m.q, m.nq);
+        const uint64_t y = csub_n(csub_n(vb[r], m.q2, m.nq2), m.q, m.nq);
+        const uint64_t prod = barrett_mul(x, y, P0.bc);
+        uint64_t d = barrett_mul(prod, scalar, P0.bc);
+        if (c && valid) {
+            d += cv[r];
+            if (d >= m.q) d -= m.q;
+        }
+        va[r] = d;
+    });
+    run_chunks<LOG_G, LOG_E, GPW, false, false>(va, lds, g, tid, m, twi, false, fin);
+    if (valid) {
+        const uint32_t ke = Gi::kt(Gi::NC - 1, tid);
+        static_for<0, E>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            st_g(out + row + ke + Gi::kr(Gi::NC - 1, r), va[r]);
+        });
+    }
+}
+
 }  // namespace axhel_dev

@@ -189,6 +189,11 @@ axhel_hip_status axhel_hip_polymul_create(const axhel_hip_polymul_job *jobs, siz
                                           axhel_hip_polymul **pm);
 axhel_hip_status axhel_hip_polymul_launch(axhel_hip_polymul *pm, void *stream);
 axhel_hip_status axhel_hip_polymul_destroy(axhel_hip_polymul *pm);
+/* Number of the product's degree groups whose fused kernel runs over stacks
+   of jobs sharing one plan (one LDS twiddle set per workgroup; same rule as
+   axhel_hip_ntt_batch_stacked_groups).  Diagnostic; no reference
+   counterpart. */
+int axhel_hip_polymul_stacked_groups(const axhel_hip_polymul *pm);
 
 /* ---------------------------------------------------------------- serving */
 

@@ -92,3 +92,51 @@ def test_polymul_is_the_negacyclic_product():
         v += 2 * bi[k + 1] if k + 1 < n else -2 * bi[0]  # -2 x^(N-1) * b: x^(N-1+j) = -x^(j-1)
         want.append(v % q)
     assert got.tolist() == want
+
+
+@pytest.mark.parametrize("logn", [15, 16])
+def test_polymul_stacked_every_config(logn):
+    """Many products per prime: the fused kernel runs over stacks of same-plan
+    jobs (one LDS twiddle set per workgroup, a padded last stack per prime).
+    Every job equals the five-call sequence; one job per prime equals the
+    oracle."""
+    torch = torch_mod()
+    n = 1 << logn
+    qs = params.rns_primes(n, 2)
+    psis = [params.find_primitive_2n_root(n, q) for q in qs]
+    plans = [ax.NTT(n, q, p) for q, p in zip(qs, psis)]
+    rng = np.random.default_rng(logn + 50)
+    try:
+        for ci in range(ax.config_count(n)):
+            ax.config_select(n, ci)
+            gpw = ax.config_get(n, ci)["row_groups"]
+            counts = [4 * gpw, 3 * gpw + 1]
+            owner = [pi for pi, c in enumerate(counts) for _ in range(c)]
+            J = len(owner)
+            A = np.stack([params.uniform(rng, n, qs[pi]) for pi in owner])
+            B = np.stack([params.uniform(rng, n, qs[pi]) for pi in owner])
+            C = np.stack([params.uniform(rng, n, qs[pi]) for pi in owner])
+            S = [int(params.uniform(rng, 1, qs[pi])[0]) for pi in owner]
+            da, db, dc = (dev(X.ravel()).view(J, n) for X in (A, B, C))
+            outs, scrs = [], []
+            for fused in (True, False):
+                out = empty_dev(J * n).view(J, n)
+                scr = empty_dev(J * n).view(J, n)
+                pm = ax.Polymul([(plans[pi], out[i], da[i], db[i], dc[i], S[i], scr[i])
+                                 for i, pi in enumerate(owner)], fused=fused)
+                if fused:
+                    assert pm.stacked_groups() == 1, (logn, ci)
+                pm.launch()
+                torch.cuda.synchronize()
+                pm.close()
+                outs.append(out)
+                scrs.append(scr)
+            assert torch.equal(outs[0], outs[1]), (logn, ci)
+            got = host(outs[0]).reshape(J, n)
+            for pi in range(len(counts)):
+                i = owner.index(pi) + counts[pi] - 1  # last job of the prime: in the padded stack
+                t = O.Tables(n, qs[pi], psis[pi])
+                assert np.array_equal(got[i], oracle_polymul(A[i], B[i], C[i], S[i], t, qs[pi])), (logn, ci, pi)
+            assert np.array_equal(host(da).reshape(J, n), A)
+    finally:
+        ax.config_select(n, -1)
