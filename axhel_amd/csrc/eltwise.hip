@@ -37,9 +37,13 @@ constexpr int kUnroll = 4;
 // Operands stream through exactly once: non-temporal loads and stores keep
 // them from churning L2 (about 12 % more HBM bandwidth on gfx950 than plain
 // accesses for these kernels).
+// Global-space (address_space 1) pointers: the accesses issue as global_*
+// instructions rather than flat_*.
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) uint64_t gword;
+typedef __attribute__((address_space(1))) u64x2 gword2;
 __device__ __forceinline__ ulonglong2 ldv(const ulonglong2 *p) {
-    const u64x2 x = __builtin_nontemporal_load(reinterpret_cast<const u64x2 *>(p));
+    const u64x2 x = __builtin_nontemporal_load((const gword2 *)p);
     ulonglong2 r;
     r.x = x.x;
     r.y = x.y;
@@ -49,10 +53,10 @@ __device__ __forceinline__ void stv(ulonglong2 *p, ulonglong2 v) {
     u64x2 x;
     x.x = v.x;
     x.y = v.y;
-    __builtin_nontemporal_store(x, reinterpret_cast<u64x2 *>(p));
+    __builtin_nontemporal_store(x, (gword2 *)p);
 }
-__device__ __forceinline__ uint64_t lds1(const uint64_t *p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ void sts1(uint64_t *p, uint64_t v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ uint64_t lds1(const uint64_t *p) { return __builtin_nontemporal_load((const gword *)p); }
+__device__ __forceinline__ void sts1(uint64_t *p, uint64_t v) { __builtin_nontemporal_store(v, (gword *)p); }
 
 // Whether the op reads a second vector operand.
 constexpr bool has_b(int op) { return op == OP_ADD_VV || op == OP_SUB_VV || op == OP_MUL || op == OP_FMA; }

@@ -66,17 +66,37 @@ struct PassArgs {
     uint32_t vec16;            // every job's src and dst are 16-byte aligned
 };
 
+// The job's descriptor: from the device table, or the inline copy in the
+// kernel arguments.  Fields are read one by one from a global-space pointer so
+// the two sources stay separate loads (a select between their addresses would
+// need generic pointers and a private copy of the kernel arguments).
 __device__ __forceinline__ NttJobDev load_job(const PassArgs &a, uint32_t limb) {
-    return a.jobs ? a.jobs[limb] : a.inline_job;
+    if (!a.jobs) return a.inline_job;
+    typedef __attribute__((address_space(1))) const NttJobDev gjob;
+    const gjob *p = (const gjob *)a.jobs + limb;
+    NttJobDev j;
+    j.src = p->src;
+    j.dst = p->dst;
+    j.tw = p->tw;
+    j.q = p->q;
+    j.ninv = p->ninv;
+    j.ninv_q = p->ninv_q;
+    j.sroot = p->sroot;
+    j.sroot_q = p->sroot_q;
+    return j;
 }
 
 __device__ __forceinline__ const uint64_t *job_src(const PassArgs &a, uint32_t limb) {
-    if (a.jobs) return a.first_pass ? a.jobs[limb].src : a.jobs[limb].dst;
-    return a.first_pass ? a.inline_job.src : a.inline_job.dst;
+    if (!a.jobs) return a.first_pass ? a.inline_job.src : a.inline_job.dst;
+    typedef __attribute__((address_space(1))) const NttJobDev gjob;
+    const gjob *p = (const gjob *)a.jobs + limb;
+    return a.first_pass ? p->src : p->dst;
 }
 
 __device__ __forceinline__ uint64_t *job_dst(const PassArgs &a, uint32_t limb) {
-    return a.jobs ? a.jobs[limb].dst : a.inline_job.dst;
+    if (!a.jobs) return a.inline_job.dst;
+    typedef __attribute__((address_space(1))) const NttJobDev gjob;
+    return ((const gjob *)a.jobs + limb)->dst;
 }
 
 template <int I, int N, typename F>
