@@ -83,14 +83,17 @@ def eltwise_leg(ax, torch, src, dst, q, reps):
     out = {}
     st = torch.cuda.current_stream()
     for name, (bpe, fn) in ops.items():
+        # reps launches back to back between one pair of events: the host's
+        # launch latency hides behind the running kernels, so this is the
+        # sustained duration of one launch
         fn()
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-        for e0, e1 in evs:
-            e0.record(st)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
             fn()
-            e1.record(st)
+        e1.record(st)
         torch.cuda.synchronize()
-        ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / reps
+        ms = e0.elapsed_time(e1) / reps
         gbs = bpe * half / (ms * 1e-3) / 1e9
         out[name] = {"elements": half, "ms": ms, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
     return out
@@ -304,18 +307,20 @@ def main():
     # per-kernel durations with HIP events on the stream the kernels run on
     stream = torch.cuda.Stream()
     kcount = batch.kernel_count()
-    reps = max(3, min(10, args.steps))
+    reps = max(5, min(20, args.steps))
     kdur = []
     with torch.cuda.stream(stream):
         for k in range(kcount):
+            # reps launches of kernel k back to back between one pair of
+            # events on its stream (sustained duration of one launch)
             batch.launch_kernel(k, stream)  # warm
-            evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-            for a, b in evs:
-                a.record(stream)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(reps):
                 batch.launch_kernel(k, stream)
-                b.record(stream)
+            b.record(stream)
             stream.synchronize()
-            kdur.append(sum(a.elapsed_time(b) for a, b in evs) / reps)
+            kdur.append(a.elapsed_time(b) / reps)
     log("kernel avg ms:", [round(x, 4) for x in kdur])
     dom = int(np.argmax(kdur))
     # Algorithmic work of one pass launch: each of the two passes runs half of
@@ -378,7 +383,7 @@ def main():
 
     elt = None
     if not args.no_eltwise:
-        elt = eltwise_leg(ax, torch, src, dst, primes[0], reps=max(3, min(10, args.steps)))
+        elt = eltwise_leg(ax, torch, src, dst, primes[0], reps=max(5, min(20, args.steps)))
         log("eltwise GB/s:", {k: round(v["GB/s"], 1) for k, v in elt.items()})
 
     del batch, ibatch
