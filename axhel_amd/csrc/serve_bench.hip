@@ -339,6 +339,7 @@ int main(int argc, char **argv) {
     const double cap = capacity(B, seconds, 32);
     axhel_hip_serve_stats cst;
     axhel_hip_server_stats(B.srv, &cst);
+    std::fprintf(stderr, "capacity: %.1f req/s over %.0f s\n", cap, seconds);
     const double cap_limbs = cst.first_arrival_to_last_completion_s > 0
                                  ? (double)cst.limbs_completed / cst.first_arrival_to_last_completion_s
                                  : 0.0;
