@@ -234,6 +234,60 @@ struct TwLds {
     }
 };
 
+// Staging of a pass's 2^LOG_G - 1 twiddle pairs into LDS by NT threads:
+// fetch() issues the global loads into registers (K per thread), so the
+// caller can issue its data loads before store() waits for them; the
+// caller's barrier then publishes the copy.
+template <int LOG_G, int NT>
+struct TwStage {
+    static constexpr uint32_t G = 1u << LOG_G;
+    static constexpr int K = (int)((G - 1u + NT - 1) / NT);
+    ulonglong2 v[K];
+
+    // ROW pass, row o of degree n: entry i is (j, lp) of tw_lds_index, read
+    // from the table at the stage base plus (o << j) + lp.
+    // Entries past the end are clamped to the last one (a duplicate load,
+    // not stored), keeping the loads branch-free so later loads need not
+    // wait for them.
+    template <bool FWD>
+    __device__ __forceinline__ void fetch_row(const ulonglong2 *tw, uint32_t n, uint32_t o, uint32_t t) {
+        static_for<0, K>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const uint32_t i = min(t + (uint32_t)k * NT, G - 2u);
+            uint32_t j, lp, base;
+            if constexpr (FWD) {
+                j = 31u - __clz(i + 1u);
+                lp = i + 1u - (1u << j);
+                base = n >> (LOG_G - j);
+            } else {
+                j = 31u - __clz(G - i - 1u);
+                lp = i - (G - (2u << j));
+                base = n + 1u - (n >> (LOG_G - 1 - j));
+            }
+            v[k] = ld_g2(tw + base + (o << j) + lp);
+        });
+    }
+
+    // COL pass: one contiguous range, entries [base, base + G - 1) with
+    // base = 1 (forward) or n + 1 - G (inverse).
+    template <bool FWD>
+    __device__ __forceinline__ void fetch_col(const ulonglong2 *tw, uint32_t n, uint32_t t) {
+        const uint32_t base = FWD ? 1u : n + 1u - G;
+        static_for<0, K>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            v[k] = ld_g2(tw + base + min(t + (uint32_t)k * NT, G - 2u));
+        });
+    }
+
+    __device__ __forceinline__ void store(ulonglong2 *tw_lds, uint32_t t) const {
+        static_for<0, K>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const uint32_t i = t + (uint32_t)k * NT;
+            if (i < G - 1u) tw_lds[i] = v[k];
+        });
+    }
+};
+
 // The butterfly chunks of one pass over registers v (chunk-0 layout on entry,
 // last-chunk layout on exit), redistributing through LDS between chunks.
 // Every thread of the workgroup must call it (it holds barriers).
@@ -331,15 +385,16 @@ ntt_pass_kernel(PassArgs a) {
         g = t / T;
     }
 
-    // Group -> (limb, group-in-limb).
+    // Group -> (limb, group-in-limb); groups_per_limb = N >> LOG_G = 2^lgpl.
+    const uint32_t lgpl = a.logn - LOG_G;
     const uint32_t gid = tile * GPW + g;
     uint32_t limb, o;
     if constexpr (MULTI) {
-        limb = gid / a.groups_per_limb;
-        o = gid % a.groups_per_limb;
+        limb = gid >> lgpl;
+        o = gid & ((1u << lgpl) - 1u);
     } else {
-        limb = (tile * GPW) / a.groups_per_limb;  // uniform over the workgroup
-        o = gid - limb * a.groups_per_limb;
+        limb = (tile * GPW) >> lgpl;  // uniform over the workgroup
+        o = gid - (limb << lgpl);
     }
     const bool valid = limb < a.num_limbs;
     const uint32_t jl = valid ? limb : 0;
@@ -358,7 +413,7 @@ ntt_pass_kernel(PassArgs a) {
         else return (o << LOG_G) | k;
     };
     // ROW tiles of one limb are GPW * G contiguous words.
-    const uint32_t tile_o0 = MULTI ? 0u : tile * GPW - limb * a.groups_per_limb;
+    const uint32_t tile_o0 = MULTI ? 0u : tile * GPW - (limb << lgpl);
 
     uint64_t v[E];
 
@@ -366,10 +421,8 @@ ntt_pass_kernel(PassArgs a) {
     // per pass, shared by every column of the limb); the barrier below the
     // data loads publishes them.
     ulonglong2 *tw_lds = reinterpret_cast<ulonglong2 *>(lds + L::WORDS);
-    if constexpr (COL) {
-        const uint32_t base = FWD ? 1u : n + 1u - (uint32_t)G;
-        for (uint32_t i = t; i < (uint32_t)G - 1u; i += NT) tw_lds[i] = ld_g2(tw + base + i);
-    }
+    TwStage<LOG_G, NT> tstage;
+    if constexpr (COL) tstage.template fetch_col<FWD>(tw, n, t);
 
     // ---- load: registers in chunk-0 layout
     constexpr bool LOAD_VIA_LDS = !COL && !FWD;  // inverse ROW: chunk 0 is thread-contiguous
@@ -422,6 +475,7 @@ ntt_pass_kernel(PassArgs a) {
 
     // ---- chunks
     if constexpr (COL) {
+        tstage.store(tw_lds, t);
         __syncthreads();
         const TwLds<LOG_G, FWD> tws{tw_lds};
         run_chunks<LOG_G, LOG_E, GPW, FWD, COL>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
@@ -488,29 +542,6 @@ ntt_pass_kernel(PassArgs a) {
     }
 }
 
-// Copies the 2^LOG_G - 1 twiddle pairs of ROW-pass row o (degree n) into
-// LDS in the TwLds order: entry i is (j, lp) of tw_lds_index, read from the
-// table at the stage base plus (o << j) + lp.  Cooperative over nthreads
-// threads; the caller's barrier publishes the copy.
-template <int LOG_G, bool FWD>
-__device__ __forceinline__ void stage_row_twiddles(ulonglong2 *tw_lds, const ulonglong2 *tw, uint32_t n, uint32_t o,
-                                                   uint32_t t, uint32_t nthreads) {
-    constexpr uint32_t G = 1u << LOG_G;
-    for (uint32_t i = t; i < G - 1u; i += nthreads) {
-        uint32_t j, lp, base;
-        if constexpr (FWD) {
-            j = 31u - __clz(i + 1u);
-            lp = i + 1u - (1u << j);
-            base = n >> (LOG_G - j);
-        } else {
-            j = 31u - __clz(G - i - 1u);
-            lp = i - (G - (2u << j));
-            base = n + 1u - (n >> (LOG_G - 1 - j));
-        }
-        tw_lds[i] = ld_g2(tw + base + (o << j) + lp);
-    }
-}
-
 // ROW pass over stacks of jobs that share one plan (same modulus and
 // twiddle table).  Workgroup (stack s, row o) transforms row o of the GPW
 // jobs jobs[s*GPW .. s*GPW+GPW-1]; an empty slot has dst == nullptr, slot 0
@@ -537,8 +568,9 @@ ntt_row_stack_kernel(PassArgs a) {
     }
     const uint32_t t = threadIdx.x;
     const uint32_t tid = t % T, g = t / T;
-    const uint32_t st = tile / a.groups_per_limb;     // stack (num_limbs = number of stacks)
-    const uint32_t o = tile - st * a.groups_per_limb;  // row
+    const uint32_t lgpl = a.logn - LOG_G;        // groups_per_limb = 2^lgpl
+    const uint32_t st = tile >> lgpl;            // stack (num_limbs = number of stacks)
+    const uint32_t o = tile & ((1u << lgpl) - 1u);  // row
     const NttJobDev J0 = a.jobs[st * GPW];              // the stack's plan
     const NttJobDev *Jg = a.jobs + st * GPW + g;
     uint64_t *dst = Jg->dst;
@@ -548,7 +580,8 @@ ntt_row_stack_kernel(PassArgs a) {
     const uint32_t n = 1u << a.logn;
 
     ulonglong2 *tw_lds = reinterpret_cast<ulonglong2 *>(lds + L::WORDS);
-    stage_row_twiddles<LOG_G, FWD>(tw_lds, J0.tw, n, o, t, NT);
+    TwStage<LOG_G, NT> tstage;
+    tstage.template fetch_row<FWD>(J0.tw, n, o, t);
 
     const uint64_t row = (uint64_t)o << LOG_G;
     uint64_t v[E];
@@ -572,6 +605,7 @@ ntt_row_stack_kernel(PassArgs a) {
                 }
             }
         }
+        tstage.store(tw_lds, t);
         __syncthreads();  // also publishes the twiddles
         const uint32_t k0 = Gm::kt(0, tid);
         static_for<0, E>([&](auto rc) {
@@ -588,6 +622,7 @@ ntt_row_stack_kernel(PassArgs a) {
         } else {
             static_for<0, E>([&](auto rc) { v[decltype(rc)::value] = 0; });
         }
+        tstage.store(tw_lds, t);
         __syncthreads();  // publishes the twiddles
     }
 

@@ -70,8 +70,9 @@ polymul_row_kernel(PolyArgs a) {
     const uint32_t t = threadIdx.x;
     const uint32_t tid = t % T, g = t / T;
     const uint32_t gid = tile * GPW + g;
-    const uint32_t limb = (tile * GPW) / a.groups_per_limb;  // uniform (groups_per_limb % GPW == 0)
-    const uint32_t o = gid - limb * a.groups_per_limb;
+    const uint32_t lgpl = a.logn - LOG_G;  // groups_per_limb = 2^lgpl
+    const uint32_t limb = (tile * GPW) >> lgpl;  // uniform (groups_per_limb % GPW == 0)
+    const uint32_t o = gid - (limb << lgpl);
     const PolyJobDev P = a.jobs[limb];
     const ModConsts m = mod_consts(P.q);
     const uint32_t n = 1u << a.logn;
@@ -170,8 +171,9 @@ polymul_row_stack_kernel(PolyArgs a) {
     }
     const uint32_t t = threadIdx.x;
     const uint32_t tid = t % T, g = t / T;
-    const uint32_t st = tile / a.groups_per_limb;
-    const uint32_t o = tile - st * a.groups_per_limb;
+    const uint32_t lgpl = a.logn - LOG_G;  // groups_per_limb = 2^lgpl
+    const uint32_t st = tile >> lgpl;
+    const uint32_t o = tile & ((1u << lgpl) - 1u);
     const PolyJobDev P0 = a.jobs[st * GPW];  // the stack's plan
     const PolyJobDev *Pg = a.jobs + st * GPW + g;
     uint64_t *out = Pg->out;
@@ -184,8 +186,9 @@ polymul_row_stack_kernel(PolyArgs a) {
 
     ulonglong2 *twf_lds = reinterpret_cast<ulonglong2 *>(lds + L::WORDS);
     ulonglong2 *twi_lds = twf_lds + (G - 1);
-    stage_row_twiddles<LOG_G, true>(twf_lds, P0.twf, n, o, t, NT);
-    stage_row_twiddles<LOG_G, false>(twi_lds, P0.twi, n, o, t, NT);
+    TwStage<LOG_G, NT> sf, si;
+    sf.template fetch_row<true>(P0.twf, n, o, t);
+    si.template fetch_row<false>(P0.twi, n, o, t);
 
     NttJobDev fin{};  // no final stage here: it belongs to the inverse COL pass
 
@@ -195,6 +198,8 @@ polymul_row_stack_kernel(PolyArgs a) {
         constexpr int r = decltype(rc)::value;
         va[r] = valid ? ld_g(a1 + row + k0 + Gf::kr(0, r)) : 0;
     });
+    sf.store(twf_lds, t);
+    si.store(twi_lds, t);
     __syncthreads();  // publishes the twiddles
     const TwLds<LOG_G, true> twf{twf_lds};
     const TwLds<LOG_G, false> twi{twi_lds};
