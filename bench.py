@@ -226,7 +226,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(limbs are independent: see the sharding note in DESIGN.md)", file=sys.stderr)
+              "(RNS limbs are independent: N GPUs would each run this batch over their share of the primes)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
