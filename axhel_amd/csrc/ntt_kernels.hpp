@@ -173,6 +173,14 @@ typedef __attribute__((address_space(1))) u64x2 gword2;
 // read through ld_g2, stay cacheable).
 __device__ __forceinline__ uint64_t ld_g(const uint64_t *p) { return __builtin_nontemporal_load((const gword *)p); }
 __device__ __forceinline__ void st_g(uint64_t *p, uint64_t v) { __builtin_nontemporal_store(v, (gword *)p); }
+// base + byte_off with a 32-bit offset: a uniform base can then stay in
+// SGPRs (the instruction's scalar base) and the offset is one VGPR.
+__device__ __forceinline__ uint64_t ld_gb(const uint64_t *base, uint32_t byte_off) {
+    return __builtin_nontemporal_load((const gword *)((const char *)base + byte_off));
+}
+__device__ __forceinline__ void st_gb(uint64_t *base, uint32_t byte_off, uint64_t v) {
+    __builtin_nontemporal_store(v, (gword *)((char *)base + byte_off));
+}
 __device__ __forceinline__ ulonglong2 ld_g2(const void *p) {
     const u64x2 x = *(const gword2 *)p;
     ulonglong2 r;
@@ -464,10 +472,20 @@ ntt_pass_kernel(PassArgs a) {
     } else {
         const uint32_t k0 = Gm::kt(0, tid);
         if (valid) {
-            static_for<0, E>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                v[r] = ld_g(src + gaddr(k0 + Gm::kr(0, r)));
-            });
+            if constexpr (COL) {
+                // word (k0 + kr) << blo | o: the register part kr << blo is
+                // uniform (scalar base), the thread part one 32-bit offset
+                const uint32_t toff = ((k0 << blo) | o) << 3;
+                static_for<0, E>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    v[r] = ld_gb(src + ((uint64_t)Gm::kr(0, r) << blo), toff);
+                });
+            } else {
+                static_for<0, E>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    v[r] = ld_g(src + gaddr(k0 + Gm::kr(0, r)));
+                });
+            }
         } else {
             static_for<0, E>([&](auto rc) { v[decltype(rc)::value] = 0; });
         }
@@ -534,10 +552,18 @@ ntt_pass_kernel(PassArgs a) {
     } else {
         if (valid) {
             const uint32_t kl = Gm::kt(NC - 1, tid);
-            static_for<0, E>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                st_g(dst + gaddr(kl + Gm::kr(NC - 1, r)), v[r]);
-            });
+            if constexpr (COL) {
+                const uint32_t toff = ((kl << blo) | o) << 3;
+                static_for<0, E>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    st_gb(dst + ((uint64_t)Gm::kr(NC - 1, r) << blo), toff, v[r]);
+                });
+            } else {
+                static_for<0, E>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    st_g(dst + gaddr(kl + Gm::kr(NC - 1, r)), v[r]);
+                });
+            }
         }
     }
 }
