@@ -160,6 +160,14 @@ struct Lds {
         if constexpr (COL) return k * GPW + g;
         else return g * ROW_STRIDE + k + (k >> LOG_E);
     }
+    // idx(g, kt | kr) for a thread part kt and a register part kr with
+    // disjoint bits (Geom's kt and kr always are): the padding then splits,
+    // k + (k >> LOG_E) = [kt + (kt >> LOG_E)] + [kr + (kr >> LOG_E)], and a
+    // compile-time kr folds into the LDS instruction's offset.
+    __device__ static uint32_t idx2(uint32_t g, uint32_t kt, uint32_t kr) {
+        if constexpr (COL) return (kt + kr) * GPW + g;
+        else return g * ROW_STRIDE + (kt + (kt >> LOG_E)) + (kr + (kr >> LOG_E));
+    }
 };
 
 // Global-memory accesses through address-space-1 pointers, so they issue as
@@ -318,12 +326,12 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
             __syncthreads();
             static_for<0, E>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
-                lds[L::idx(g, kp + Gm::kr(c - 1, r))] = v[r];
+                lds[L::idx2(g, kp, Gm::kr(c - 1, r))] = v[r];
             });
             __syncthreads();
             static_for<0, E>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
-                v[r] = lds[L::idx(g, ktid + Gm::kr(c, r))];
+                v[r] = lds[L::idx2(g, ktid, Gm::kr(c, r))];
             });
         }
 
@@ -467,7 +475,7 @@ ntt_pass_kernel(PassArgs a) {
         const uint32_t k0 = Gm::kt(0, tid);
         static_for<0, E>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
-            v[r] = lds[L::idx(g, k0 + Gm::kr(0, r))];
+            v[r] = lds[L::idx2(g, k0, Gm::kr(0, r))];
         });
     } else {
         const uint32_t k0 = Gm::kt(0, tid);
@@ -518,7 +526,7 @@ ntt_pass_kernel(PassArgs a) {
         __syncthreads();
         static_for<0, E>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
-            lds[L::idx(g, kl + Gm::kr(NC - 1, r))] = v[r];
+            lds[L::idx2(g, kl, Gm::kr(NC - 1, r))] = v[r];
         });
         __syncthreads();
         if constexpr (!MULTI) {
@@ -618,16 +626,16 @@ ntt_row_stack_kernel(PassArgs a) {
             if (a.vec16) {
 #pragma unroll
                 for (int i = 0; i < E / 2; ++i) {
-                    const uint32_t k = 2u * (tid + (uint32_t)i * T);
-                    const ulonglong2 x2 = ld_nt2(src + row + k);
-                    lds[L::idx(g, k)] = x2.x;
-                    lds[L::idx(g, k + 1)] = x2.y;
+                    const uint32_t kr = 2u * (uint32_t)i * T;  // disjoint from the thread part 2 tid (+1)
+                    const ulonglong2 x2 = ld_nt2(src + row + 2u * tid + kr);
+                    lds[L::idx2(g, 2u * tid, kr)] = x2.x;
+                    lds[L::idx2(g, 2u * tid + 1u, kr)] = x2.y;
                 }
             } else {
 #pragma unroll
                 for (int i = 0; i < E; ++i) {
-                    const uint32_t k = tid + (uint32_t)i * T;
-                    lds[L::idx(g, k)] = ld_g(src + row + k);
+                    const uint32_t kr = (uint32_t)i * T;
+                    lds[L::idx2(g, tid, kr)] = ld_g(src + row + tid + kr);
                 }
             }
         }
@@ -636,7 +644,7 @@ ntt_row_stack_kernel(PassArgs a) {
         const uint32_t k0 = Gm::kt(0, tid);
         static_for<0, E>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
-            v[r] = lds[L::idx(g, k0 + Gm::kr(0, r))];
+            v[r] = lds[L::idx2(g, k0, Gm::kr(0, r))];
         });
     } else {
         const uint32_t k0 = Gm::kt(0, tid);
@@ -669,24 +677,24 @@ ntt_row_stack_kernel(PassArgs a) {
         __syncthreads();
         static_for<0, E>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
-            lds[L::idx(g, kl + Gm::kr(NC - 1, r))] = v[r];
+            lds[L::idx2(g, kl, Gm::kr(NC - 1, r))] = v[r];
         });
         __syncthreads();
         if (valid) {
             if (a.vec16) {
 #pragma unroll
                 for (int i = 0; i < E / 2; ++i) {
-                    const uint32_t k = 2u * (tid + (uint32_t)i * T);
+                    const uint32_t kr = 2u * (uint32_t)i * T;
                     ulonglong2 x2;
-                    x2.x = lds[L::idx(g, k)];
-                    x2.y = lds[L::idx(g, k + 1)];
-                    st_g2(dst + row + k, x2);
+                    x2.x = lds[L::idx2(g, 2u * tid, kr)];
+                    x2.y = lds[L::idx2(g, 2u * tid + 1u, kr)];
+                    st_g2(dst + row + 2u * tid + kr, x2);
                 }
             } else {
 #pragma unroll
                 for (int i = 0; i < E; ++i) {
-                    const uint32_t k = tid + (uint32_t)i * T;
-                    st_g(dst + row + k, lds[L::idx(g, k)]);
+                    const uint32_t kr = (uint32_t)i * T;
+                    st_g(dst + row + tid + kr, lds[L::idx2(g, tid, kr)]);
                 }
             }
         }
