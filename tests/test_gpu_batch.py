@@ -170,3 +170,33 @@ def test_sparse_plans_do_not_stack():
     torch.cuda.synchronize()
     assert int(out.abs().sum()) == 0
     b.close()
+
+
+def test_empty_batch_and_empty_product():
+    torch = torch_mod()
+    b = ax.Batch([])
+    assert b.kernel_count() == 0 and b.stacked_groups() == 0
+    b.launch()
+    b.launch_eager()
+    torch.cuda.synchronize()
+    b.close()
+    pm = ax.Polymul([])
+    assert pm.stacked_groups() == 0
+    pm.launch()
+    torch.cuda.synchronize()
+    pm.close()
+
+
+def test_batch_rejects_host_pointers():
+    """Batch jobs must be device memory (the plain calls stage host buffers;
+    batches do not)."""
+    torch = torch_mod()
+    n = 1 << 12
+    q = params.find_ntt_prime(n, 50)
+    plan = ax.NTT(n, q, params.find_primitive_2n_root(n, q))
+    d = torch.zeros(n, dtype=torch.int64, device="cuda")
+    h = np.zeros(n, dtype=np.uint64)
+    with pytest.raises(ax.AxhelError):
+        ax.Batch([(plan, d, h.ctypes.data, False, 1)])
+    with pytest.raises(ax.AxhelError):
+        ax.Batch([(plan, h.ctypes.data, d, False, 1)])
