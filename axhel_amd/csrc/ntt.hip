@@ -12,6 +12,7 @@
 #include <new>
 #include <string>
 
+#include "eltwise_host.hpp"
 #include "ntt_host.hpp"
 #include "polymul_kernels.hpp"
 
@@ -38,37 +39,68 @@ uint64_t powmod(uint64_t b, uint64_t e, uint64_t q) {
 
 uint64_t shoup_quotient(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 64) / q); }
 
-// Twiddle tables of ntt-tables.cpp:81-140: fwd[i] = psi^brv(i);
-// inv[0] = {0, 0}, inv[i] = psi^-(brv(i-1)+1); n^-1 = n^(q-2).  Bit reversal
-// is computed directly rather than by the reference's incremental counter;
-// the index sequence is the same.
-void gen_tables(uint32_t logn, uint64_t q, uint64_t psi, std::vector<uint64_t> &fwd, std::vector<uint64_t> &inv,
-                uint64_t &ninv) {
+}  // namespace
+}  // namespace axhel_host
+
+namespace axhel_dev {
+
+// psi^e mod q by square-and-multiply; base < q.  barrett_mul is the exact
+// product mod q for operands below q, so every power equals the host's
+// (u128)a * b % q chain.
+__device__ __forceinline__ uint64_t powmod_dev(uint64_t b, uint64_t e, const Barrett &bc) {
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = barrett_mul(r, b, bc);
+        e >>= 1;
+        if (e) b = barrett_mul(b, b, bc);
+    }
+    return r;
+}
+
+// floor(w * 2^64 / q) for w < q < 2^62: restoring long division of w * 2^64
+// by q, one quotient bit per step (the remainder stays below 2q < 2^63).
+__device__ __forceinline__ uint64_t shoup_quotient_dev(uint64_t w, uint64_t q) {
+    uint64_t rem = w, quo = 0;
+    for (int b = 0; b < 64; ++b) {
+        rem <<= 1;
+        quo <<= 1;
+        if (rem >= q) {
+            rem -= q;
+            quo |= 1;
+        }
+    }
+    return quo;
+}
+
+// Twiddle tables of ntt-tables.cpp:81-140, one entry per thread:
+// fwd[i] = psi^brv(i); inv[0] = {0, 0}, inv[i] = psi^-(brv(i-1)+1), each with
+// its Shoup quotient.  The reference walks the bit-reversed index
+// incrementally and builds powers sequentially; entry i here is computed
+// directly, the same value.  psi and ipsi are reduced mod q.
+__global__ void gen_tables_kernel(ulonglong2 *fwd, ulonglong2 *inv, uint32_t logn, uint64_t q, uint64_t psi,
+                                  uint64_t ipsi, Barrett bc) {
     const uint64_t n = 1ull << logn;
-    auto brv = [logn](uint64_t i) -> uint64_t {
-        uint64_t r = 0;
-        for (uint32_t b = 0; b < logn; ++b) r |= ((i >> b) & 1ull) << (logn - 1 - b);
-        return r;
-    };
-    std::vector<uint64_t> pw(n);
-    pw[0] = 1 % q;
-    for (uint64_t k = 1; k < n; ++k) pw[k] = mulmod(pw[k - 1], psi, q);
-    fwd.assign(2 * n, 0);
-    for (uint64_t i = 0; i < n; ++i) {
-        const uint64_t w = pw[brv(i)];
-        fwd[2 * i] = w;
-        fwd[2 * i + 1] = shoup_quotient(w, q);
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t w = powmod_dev(psi, __brevll(i) >> (64 - logn), bc);
+    fwd[i] = make_ulonglong2(w, shoup_quotient_dev(w, q));
+    if (i == 0) {
+        inv[0] = make_ulonglong2(0, 0);
+    } else {
+        const uint64_t wi = powmod_dev(ipsi, (__brevll(i - 1) >> (64 - logn)) + 1, bc);
+        inv[i] = make_ulonglong2(wi, shoup_quotient_dev(wi, q));
     }
-    const uint64_t ipsi = powmod(psi, q - 2, q);
-    pw[0] = 1 % q;
-    for (uint64_t k = 1; k < n; ++k) pw[k] = mulmod(pw[k - 1], ipsi, q);
-    inv.assign(2 * n, 0);
-    for (uint64_t i = 1; i < n; ++i) {
-        const uint64_t w = pw[brv(i - 1) + 1];
-        inv[2 * i] = w;
-        inv[2 * i + 1] = shoup_quotient(w, q);
-    }
-    ninv = powmod(n % q, q - 2, q);
+}
+
+}  // namespace axhel_dev
+
+namespace axhel_host {
+namespace {
+
+uint64_t brv_host(uint64_t i, uint32_t logn) {
+    uint64_t r = 0;
+    for (uint32_t b = 0; b < logn; ++b) r |= ((i >> b) & 1ull) << (logn - 1 - b);
+    return r;
 }
 
 // ------------------------------------------------------------ configurations
@@ -368,6 +400,8 @@ hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool no
         } else {
             blocks = (uint64_t)count * a.groups_per_limb / k.gpw;
         }
+        // the kernels index groups with 32-bit arithmetic (tile * GPW + g)
+        if (blocks == 0 || blocks * (uint64_t)k.gpw > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
         void *args[] = {&a};
         e = hipLaunchKernel(k.fn, dim3((unsigned)blocks), dim3(k.threads), args, k.lds_bytes, stream);
         if (e != hipSuccess) return e;
@@ -383,10 +417,12 @@ using namespace axhel_host;
 
 namespace {
 
-axhel_hip_status finish_plan(axhel_hip_plan *p, axhel_hip_plan **out) {
-    // ntt-native.cpp:109-113: the final inverse stage's scaled root, computed
-    // once per plan instead of once per call (same value).
-    const uint64_t wl = p->h_inv[2 * (p->n - 1)];
+// Finishes a plan whose ninv, ninv_q are set: the final inverse stage's
+// scaled root from wl = inv_root_powers[N-1].operand (ntt-native.cpp:109-113,
+// computed once per plan instead of once per call; same value), then the
+// device tables: uploaded from h_fwd/h_inv when the caller supplied them
+// (external tables), else generated on the device from psi.
+axhel_hip_status finish_plan(axhel_hip_plan *p, uint64_t wl, uint64_t psi, uint64_t ipsi, axhel_hip_plan **out) {
     uint64_t s = wl * p->ninv - (uint64_t)(((u128)wl * p->ninv_q) >> 64) * p->q;
     if (s >= p->q) s -= p->q;
     p->sroot = s;
@@ -394,10 +430,19 @@ axhel_hip_status finish_plan(axhel_hip_plan *p, axhel_hip_plan **out) {
     hipError_t e = hipGetDevice(&p->device);
     if (e == hipSuccess) e = hipMalloc(&p->d_fwd, p->n * sizeof(ulonglong2));
     if (e == hipSuccess) e = hipMalloc(&p->d_inv, p->n * sizeof(ulonglong2));
-    if (e == hipSuccess)
-        e = hipMemcpy(p->d_fwd, p->h_fwd.data(), p->n * sizeof(ulonglong2), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(p->d_inv, p->h_inv.data(), p->n * sizeof(ulonglong2), hipMemcpyHostToDevice);
+    if (!p->h_fwd.empty()) {
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_fwd, p->h_fwd.data(), p->n * sizeof(ulonglong2), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_inv, p->h_inv.data(), p->n * sizeof(ulonglong2), hipMemcpyHostToDevice);
+    } else if (e == hipSuccess) {
+        const unsigned threads = 256;
+        const unsigned blocks = (unsigned)((p->n + threads - 1) / threads);
+        hipLaunchKernelGGL(axhel_dev::gen_tables_kernel, dim3(blocks), dim3(threads), 0, nullptr, p->d_fwd, p->d_inv,
+                           p->logn, p->q, psi, ipsi, barrett_for(p->q));
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    }
     if (e != hipSuccess) {
         set_last_error("plan upload", e);
         (void)hipFree(p->d_fwd);
@@ -450,9 +495,15 @@ axhel_hip_status axhel_hip_ntt_plan_create(uint64_t degree, uint64_t modulus, ui
     p->n = degree;
     p->logn = logn;
     p->q = modulus;
-    gen_tables(logn, modulus, root_of_unity, p->h_fwd, p->h_inv, p->ninv);
+    // scalars on the host: psi^-1 = psi^(q-2), n^-1 = n^(q-2) (ntt-tables.cpp
+    // conventions), and inv_root_powers[N-1] for the final-stage root; the
+    // tables themselves are generated on the device
+    const uint64_t psi = root_of_unity % modulus;
+    const uint64_t ipsi = powmod(psi, modulus - 2, modulus);
+    p->ninv = powmod(degree % modulus, modulus - 2, modulus);
     p->ninv_q = shoup_quotient(p->ninv, modulus);
-    return finish_plan(p, plan);
+    const uint64_t wl = degree > 1 ? powmod(ipsi, brv_host(degree - 2, logn) + 1, modulus) : 0;
+    return finish_plan(p, wl, psi, ipsi, plan);
 }
 
 axhel_hip_status axhel_hip_ntt_plan_create_external(uint64_t degree, uint64_t modulus, const uint64_t *root_powers,
@@ -473,7 +524,7 @@ axhel_hip_status axhel_hip_ntt_plan_create_external(uint64_t degree, uint64_t mo
     p->h_inv.assign(inv_root_powers, inv_root_powers + 2 * degree);
     p->ninv = inv_degree_operand;
     p->ninv_q = inv_degree_quotient;
-    return finish_plan(p, plan);
+    return finish_plan(p, inv_root_powers[2 * (degree - 1)], 0, 0, plan);
 }
 
 axhel_hip_status axhel_hip_ntt_plan_destroy(axhel_hip_plan *plan) {
@@ -490,8 +541,10 @@ uint64_t axhel_hip_ntt_plan_modulus(const axhel_hip_plan *plan) { return plan ? 
 axhel_hip_status axhel_hip_ntt_plan_tables(const axhel_hip_plan *plan, uint64_t *root_powers,
                                            uint64_t *inv_root_powers, uint64_t *inv_degree) {
     if (!plan) return AXHEL_HIP_INVALID_ARGUMENT;
-    if (root_powers) std::memcpy(root_powers, plan->h_fwd.data(), plan->h_fwd.size() * sizeof(uint64_t));
-    if (inv_root_powers) std::memcpy(inv_root_powers, plan->h_inv.data(), plan->h_inv.size() * sizeof(uint64_t));
+    // the device tables are the plan's (host copies exist only for external tables)
+    if (root_powers) AXHEL_HIP_TRY(hipMemcpy(root_powers, plan->d_fwd, plan->n * sizeof(ulonglong2), hipMemcpyDeviceToHost));
+    if (inv_root_powers)
+        AXHEL_HIP_TRY(hipMemcpy(inv_root_powers, plan->d_inv, plan->n * sizeof(ulonglong2), hipMemcpyDeviceToHost));
     if (inv_degree) {
         inv_degree[0] = plan->ninv;
         inv_degree[1] = plan->ninv_q;

@@ -112,6 +112,7 @@ hipError_t issue(axhel_hip_polymul *pm, hipStream_t st) {
                 a.num_limbs = g.fstacks;
                 blocks = (uint64_t)g.fstacks * a.groups_per_limb;
             }
+            if (blocks == 0 || blocks * (uint64_t)fk.gpw > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
             void *args[] = {&a};
             if ((e = hipLaunchKernel(fk.fn, dim3((unsigned)blocks), dim3(fk.threads), args, fk.lds_bytes, st)))
                 return e;

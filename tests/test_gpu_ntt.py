@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import axhel_amd as ax
+from axhel_amd import params
 from oracle import oracle as O
 from tests.helpers import NTT_DEGREES, MODULUS_BITS, SEED, dev, empty_dev, host, ntt_params
 
@@ -159,3 +160,23 @@ def test_library_loaded_before_torch_leaves_torch_a_gpu():
             "assert 'torch' in sys.modules; import torch; assert torch.cuda.is_available(); print('ok')")
     r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=180)
     assert r.returncode == 0 and "ok" in r.stdout, r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("logn", [1, 2, 3, 5, 8, 12, 13, 14, 16, 17])
+def test_device_generated_tables_equal_the_oracle(logn):
+    """Plan tables are generated on the device (powers by square-and-multiply,
+    Shoup quotients by long division): every word equals the oracle's tables
+    (ntt-tables.cpp:81-140 restated), for moduli of 20..62 bits, and a root
+    given above q gives the same tables as its residue."""
+    n = 1 << logn
+    for bits in (20, 41, 50, 62):
+        if (1 << bits) <= 2 * n:
+            continue
+        q = params.find_ntt_prime(n, bits)
+        psi = params.find_primitive_2n_root(n, q)
+        t = O.Tables(n, q, psi)
+        for root in (psi, psi + q):
+            fwd, inv, ni = ax.NTT(n, q, root).tables()
+            assert np.array_equal(fwd, t.fwd), (logn, bits)
+            assert np.array_equal(inv, t.inv), (logn, bits)
+            assert np.array_equal(ni, t.ninv), (logn, bits)
