@@ -135,6 +135,11 @@ SIGNATURES = {
     "axhel_hip_polymul_launch": (_st, [_p, _p]),
     "axhel_hip_polymul_destroy": (_st, [_p]),
     "axhel_hip_polymul_stacked_groups": (ctypes.c_int, [_p]),
+    "axhel_hip_rns_create": (_st, [_u64, _p, _p, ctypes.c_uint32, ctypes.POINTER(_p)]),
+    "axhel_hip_rns_destroy": (_st, [_p]),
+    "axhel_hip_rns_plan": (_p, [_p, ctypes.c_uint32]),
+    "axhel_hip_rns_ntt": (_st, [_p, _p, _p, _u64, ctypes.c_int, _u64, _p]),
+    "axhel_hip_rns_eltwise": (_st, [_p, ctypes.c_int, _p, _p, _p, _p, _u64, _u64, _u64, _p]),
     "axhel_hip_server_create": (_st, [ctypes.POINTER(ServeConfig), ctypes.POINTER(_p)]),
     "axhel_hip_server_destroy": (_st, [_p]),
     "axhel_hip_server_plan": (_st, [_p, _u64, _u64, _u64, ctypes.POINTER(_p)]),
@@ -383,6 +388,49 @@ def butterfly_peak():
 
 
 # ------------------------------------------------------- polynomial product
+RNS_ANY_INPUT = (1 << 64) - 1
+
+
+class RNS:
+    """RNS context (axhel_hip_rns_*): degree N, L moduli (roots optional),
+    operating on device tensors of count x L x N words."""
+
+    def __init__(self, degree, moduli, roots=None):
+        moduli = [int(q) for q in moduli]
+        qa = (_u64 * len(moduli))(*moduli)
+        ra = None if roots is None else (_u64 * len(moduli))(*[int(r) for r in roots])
+        h = _p()
+        _check(_load().axhel_hip_rns_create(degree, qa, ra, len(moduli), ctypes.byref(h)), "rns_create")
+        self._h = h
+        self.degree, self.moduli = degree, moduli
+
+    def ntt(self, result, operand, count, inverse=False, output_mod_factor=1, stream=None):
+        _check(_load().axhel_hip_rns_ntt(self._h, _ptr(result), _ptr(operand), count, 1 if inverse else 0,
+                                         output_mod_factor, _stream_ptr(stream)), "rns_ntt")
+
+    def eltwise(self, op, result, operand1, operand2=None, scalars=None, count=1, input_mod_factor=1,
+                output_mod_factor=1, stream=None):
+        sa = None if scalars is None else (_u64 * len(self.moduli))(*[int(x) for x in scalars])
+        _check(_load().axhel_hip_rns_eltwise(self._h, op, _ptr(result), _ptr(operand1),
+                                             None if operand2 is None else _ptr(operand2), sa, count,
+                                             input_mod_factor, output_mod_factor, _stream_ptr(stream)),
+               "rns_eltwise")
+
+    def plan_handle(self, limb):
+        return _load().axhel_hip_rns_plan(self._h, limb)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _load().axhel_hip_rns_destroy(self._h)
+            self._h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Polymul:
     """Batched RNS polynomial product: per job (ntt, result, a, b, addend or
     None, scalar, scratch) computes InverseNTT(FMA(Mul(NTT(a), NTT(b)), scalar,

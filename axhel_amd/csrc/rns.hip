@@ -1,0 +1,316 @@
+// RNS context: N and L primes with resident plans, and calls over whole RNS
+// polynomials stored [count][L][N] in device memory.  The reference leaves
+// the limb loop to its caller (one ComputeForward / Eltwise*Mod call per
+// limb, integration/seal/SEAL-4.4.5-AXHEL.patch:185-226); here one call
+// covers count x L limbs.
+//
+// Each call builds its job table on the device (a fill kernel into
+// stream-ordered scratch from hipMallocAsync, released with hipFreeAsync on
+// the same stream), so calls on different streams never share scratch and
+// no table crosses PCIe.  The transforms then run through the batched
+// kernels, the ROW pass stacked over polynomials when the stacks fill.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "eltwise_host.hpp"
+#include "ntt_host.hpp"
+
+using axhel_dev::Barrett;
+using axhel_dev::NttJobDev;
+using namespace axhel_host;
+
+namespace {
+
+constexpr uint32_t kMaxLimbs = AXHEL_HIP_RNS_MAX_LIMBS;  // per-limb scalars travel in the fill kernel's arguments
+
+struct RnsLimbDev {
+    const ulonglong2 *twf, *twi;
+    uint64_t q, ninv, ninv_q, sroot, sroot_q;
+    Barrett bc;
+};
+
+}  // namespace
+
+struct axhel_hip_rns {
+    uint64_t n = 0;
+    uint32_t logn = 0;
+    uint32_t limbs = 0;
+    std::vector<axhel_hip_plan *> plans;
+    RnsLimbDev *d_info = nullptr;
+};
+
+namespace {
+
+typedef unsigned __int128 u128;
+
+uint64_t mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)(((u128)a * b) % q); }
+
+uint64_t powmod(uint64_t b, uint64_t e, uint64_t q) {
+    uint64_t r = 1 % q;
+    b %= q;
+    while (e) {
+        if (e & 1) r = mulmod(r, b, q);
+        e >>= 1;
+        if (e) b = mulmod(b, b, q);
+    }
+    return r;
+}
+
+// The reference tests' root rule (parameters.hpp:110-122): the first
+// a = 2, 3, ... whose a^((q-1)/2N) has order exactly 2N.
+bool find_root(uint64_t n, uint64_t q, uint64_t *root) {
+    if ((q - 1) % (2 * n) != 0) return false;
+    const uint64_t e = (q - 1) / (2 * n);
+    for (uint64_t a = 2; a < 100000 && a < q; ++a) {
+        const uint64_t psi = powmod(a, e, q);
+        if (psi != 1 && powmod(psi, n, q) == q - 1 && powmod(psi, 2 * n, q) == 1) {
+            *root = psi;
+            return true;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ NttJobDev job_for(const RnsLimbDev &L, const uint64_t *src, uint64_t *dst, bool inverse) {
+    NttJobDev j;
+    j.src = src;
+    j.dst = dst;
+    j.tw = inverse ? L.twi : L.twf;
+    j.q = L.q;
+    j.ninv = L.ninv;
+    j.ninv_q = L.ninv_q;
+    j.sroot = L.sroot;
+    j.sroot_q = L.sroot_q;
+    return j;
+}
+
+// Job tables of one transform call.  jobs: limb-major, entry l*count + p is
+// limb l of polynomial p (so jobs sharing a plan are adjacent).  stacks (if
+// non-null): per limb, ceil(count/gpw) stacks of gpw slots over polynomials
+// s*gpw + g, empty slots (dst null) past count.
+__global__ void rns_fill_ntt(NttJobDev *jobs, NttJobDev *stacks, const RnsLimbDev *info, uint64_t *res,
+                             const uint64_t *op, uint32_t count, uint32_t limbs, uint64_t n, int inverse,
+                             uint32_t slots_per_limb) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nj = (uint64_t)count * limbs;
+    if (t < nj) {
+        const uint32_t l = (uint32_t)(t / count), p = (uint32_t)(t % count);
+        const uint64_t off = ((uint64_t)p * limbs + l) * n;
+        jobs[t] = job_for(info[l], op + off, res + off, inverse != 0);
+    }
+    if (stacks && t < (uint64_t)slots_per_limb * limbs) {
+        const uint32_t l = (uint32_t)(t / slots_per_limb), p = (uint32_t)(t % slots_per_limb);
+        if (p < count) {
+            const uint64_t off = ((uint64_t)p * limbs + l) * n;
+            stacks[t] = job_for(info[l], op + off, res + off, inverse != 0);
+        } else {
+            stacks[t] = job_for(info[l], nullptr, nullptr, inverse != 0);
+        }
+    }
+}
+
+struct EltFillArgs {
+    uint64_t s[kMaxLimbs];  // per-limb scalar (already reduced as the op requires)
+};
+
+__global__ void rns_fill_elt(EltLimbDev *out, const RnsLimbDev *info, uint64_t *r, const uint64_t *a,
+                             const uint64_t *b, uint32_t count, uint32_t limbs, uint64_t n, EltFillArgs sc) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)count * limbs) return;
+    const uint32_t l = (uint32_t)(t % limbs);
+    EltLimbDev d;
+    d.r = r + t * n;  // t = p * limbs + l: polynomial-major, as stored
+    d.a = a + t * n;
+    d.b = b ? b + t * n : nullptr;
+    d.s = sc.s[l];
+    d.bc = info[l].bc;
+    out[t] = d;
+}
+
+bool device_ptr(const void *p) { return p && is_device_pointer(p); }
+
+}  // namespace
+
+extern "C" {
+
+axhel_hip_status axhel_hip_rns_create(uint64_t degree, const uint64_t *moduli, const uint64_t *roots, uint32_t limbs,
+                                      axhel_hip_rns **ctx) {
+    if (!ctx || !moduli || limbs == 0 || limbs > kMaxLimbs) return AXHEL_HIP_INVALID_ARGUMENT;
+    *ctx = nullptr;
+    axhel_hip_rns *c = new (std::nothrow) axhel_hip_rns;
+    if (!c) return AXHEL_HIP_NO_MEMORY;
+    c->n = degree;
+    c->limbs = limbs;
+    std::vector<RnsLimbDev> info(limbs);
+    axhel_hip_status st = AXHEL_HIP_OK;
+    for (uint32_t l = 0; l < limbs && st == AXHEL_HIP_OK; ++l) {
+        uint64_t psi = roots ? roots[l] : 0;
+        if (!roots && !find_root(degree, moduli[l], &psi)) {
+            set_last_error("rns create: no primitive 2N-th root for a modulus", hipErrorInvalidValue);
+            st = AXHEL_HIP_INVALID_ARGUMENT;
+            break;
+        }
+        axhel_hip_plan *p = nullptr;
+        st = axhel_hip_ntt_plan_create(degree, moduli[l], psi, &p);
+        if (st != AXHEL_HIP_OK) break;
+        c->plans.push_back(p);
+        c->logn = p->logn;
+        info[l] = RnsLimbDev{p->d_fwd, p->d_inv, p->q, p->ninv, p->ninv_q, p->sroot, p->sroot_q, barrett_for(p->q)};
+    }
+    if (st == AXHEL_HIP_OK) {
+        hipError_t e = hipMalloc(&c->d_info, limbs * sizeof(RnsLimbDev));
+        if (e == hipSuccess) e = hipMemcpy(c->d_info, info.data(), limbs * sizeof(RnsLimbDev), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            set_last_error("rns create", e);
+            st = e == hipErrorOutOfMemory ? AXHEL_HIP_NO_MEMORY : AXHEL_HIP_HIP_ERROR;
+        }
+    }
+    if (st != AXHEL_HIP_OK) {
+        axhel_hip_rns_destroy(c);
+        return st;
+    }
+    *ctx = c;
+    return AXHEL_HIP_OK;
+}
+
+axhel_hip_status axhel_hip_rns_destroy(axhel_hip_rns *ctx) {
+    if (!ctx) return AXHEL_HIP_OK;
+    for (auto *p : ctx->plans) axhel_hip_ntt_plan_destroy(p);
+    (void)hipFree(ctx->d_info);
+    delete ctx;
+    return AXHEL_HIP_OK;
+}
+
+const axhel_hip_plan *axhel_hip_rns_plan(const axhel_hip_rns *ctx, uint32_t limb) {
+    return ctx && limb < ctx->limbs ? ctx->plans[limb] : nullptr;
+}
+
+axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const uint64_t *operand, uint64_t count,
+                                   int inverse, uint64_t output_mod_factor, void *stream) {
+    if (!ctx || (inverse != 0 && inverse != 1)) return AXHEL_HIP_INVALID_ARGUMENT;
+    if (count == 0) return AXHEL_HIP_OK;
+    if (!device_ptr(result) || !device_ptr(operand) || count > 0x7FFFFFFFull / ctx->limbs)
+        return AXHEL_HIP_INVALID_ARGUMENT;
+    // lazy only for output factor 4 (forward) / 2 (inverse), as ntt.cpp:173-204
+    const bool normalize = inverse ? output_mod_factor != 2 : output_mod_factor != 4;
+    const hipStream_t st = as_stream(stream);
+    const NttConfig &cfg = config_for(ctx->logn);
+    AXHEL_HIP_TRY(prepare_config(cfg));
+    const uint32_t cnt = (uint32_t)count, L = ctx->limbs;
+    const uint64_t nj = (uint64_t)cnt * L;
+    // stacked ROW pass when padding each limb's polynomials to whole stacks
+    // leaves at most one slot in eight empty (the batch rule)
+    const int gpw = stack_gpw(ctx->logn);
+    uint32_t slots_per_limb = 0;
+    if (gpw) {
+        const uint64_t spl = (cnt + (uint64_t)gpw - 1) / gpw * gpw;
+        if ((spl * L - nj) * 8 <= spl * L) slots_per_limb = (uint32_t)spl;
+    }
+    const uint64_t entries = nj + (uint64_t)slots_per_limb * L;
+    void *scratch = nullptr;
+    AXHEL_HIP_TRY(hipMallocAsync(&scratch, entries * sizeof(NttJobDev), st));
+    NttJobDev *jobs = (NttJobDev *)scratch;
+    NttJobDev *stacks = slots_per_limb ? jobs + nj : nullptr;
+    const uint64_t fill = std::max<uint64_t>(nj, (uint64_t)slots_per_limb * L);
+    hipLaunchKernelGGL(rns_fill_ntt, dim3((unsigned)((fill + 255) / 256)), dim3(256), 0, st, jobs, stacks,
+                       ctx->d_info, result, operand, cnt, L, ctx->n, inverse, slots_per_limb);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        static const NttJobDev none{};
+        StackTable tab;
+        if (stacks) {
+            tab.jobs = stacks;
+            tab.count = slots_per_limb * L / gpw;
+            tab.gpw = gpw;
+        }
+        const bool vec16 = aligned16(result) && aligned16(operand);
+        e = launch_ntt(cfg, ctx->logn, inverse != 0, normalize, jobs, none, (uint32_t)nj, st, -1, vec16,
+                       stacks ? &tab : nullptr);
+    }
+    const hipError_t fe = hipFreeAsync(scratch, st);
+    AXHEL_HIP_TRY(e);
+    AXHEL_HIP_TRY(fe);
+    return AXHEL_HIP_OK;
+}
+
+axhel_hip_status axhel_hip_rns_eltwise(axhel_hip_rns *ctx, int op, uint64_t *result, const uint64_t *operand1,
+                                       const uint64_t *operand2, const uint64_t *scalars, uint64_t count,
+                                       uint64_t input_mod_factor, uint64_t output_mod_factor, void *stream) {
+    if (!ctx) return AXHEL_HIP_INVALID_ARGUMENT;
+    if (count == 0) return AXHEL_HIP_OK;
+    if (!device_ptr(result) || !device_ptr(operand1) || (operand2 && !device_ptr(operand2)) ||
+        count > 0x7FFFFFFFull / ctx->limbs)
+        return AXHEL_HIP_INVALID_ARGUMENT;
+    const uint32_t L = ctx->limbs;
+    const hipStream_t st = as_stream(stream);
+    EltFillArgs sc{};
+    int eop = 0, f = 1;
+    const uint64_t *b = nullptr;
+    switch (op) {
+        case AXHEL_HIP_OP_ADD:
+        case AXHEL_HIP_OP_SUB:
+            if (!operand2) return AXHEL_HIP_INVALID_ARGUMENT;
+            eop = op == AXHEL_HIP_OP_ADD ? ELT_ADD_VV : ELT_SUB_VV;
+            b = operand2;
+            break;
+        case AXHEL_HIP_OP_MUL:  // eltwise-mul-mod.cpp:81-94: F in {1, 2, 4}
+            if (!operand2) return AXHEL_HIP_INVALID_ARGUMENT;
+            if (input_mod_factor != 1 && input_mod_factor != 2 && input_mod_factor != 4)
+                return AXHEL_HIP_UNSUPPORTED_FACTOR;
+            eop = ELT_MUL;
+            f = (int)input_mod_factor;
+            b = operand2;
+            break;
+        case AXHEL_HIP_OP_FMA: {  // eltwise-fma-mod.cpp:96-112: F in {1, 2, 4, 8}, addend nullable
+            if (!scalars) return AXHEL_HIP_INVALID_ARGUMENT;
+            if (input_mod_factor != 1 && input_mod_factor != 2 && input_mod_factor != 4 && input_mod_factor != 8)
+                return AXHEL_HIP_UNSUPPORTED_FACTOR;
+            eop = operand2 ? ELT_FMA : ELT_FMA_NULL;
+            f = (int)input_mod_factor;
+            b = operand2;
+            for (uint32_t l = 0; l < L; ++l) {  // ReduceInputNative<F>(scalar), eltwise-fma-mod.cpp:32
+                const uint64_t q = ctx->plans[l]->q;
+                uint64_t s = scalars[l];
+                if (f == 8 && s >= (q << 2)) s -= q << 2;
+                if (f >= 4 && s >= (q << 1)) s -= q << 1;
+                if (f >= 2 && s >= q) s -= q;
+                sc.s[l] = s;
+            }
+            break;
+        }
+        case AXHEL_HIP_OP_REDUCE: {  // branch order of eltwise-reduce-mod.cpp:32-128
+            const uint64_t fi = input_mod_factor, fo = output_mod_factor;
+            if (fi == 1 || (fo == 2 && fi <= 2)) {
+                if (result == operand1) return AXHEL_HIP_OK;
+                AXHEL_HIP_TRY(hipMemcpyAsync(result, operand1, count * L * ctx->n * sizeof(uint64_t),
+                                             hipMemcpyDeviceToDevice, st));
+                return AXHEL_HIP_OK;
+            }
+            if (fi == 2 && fo == 1) eop = ELT_RED_2_1;
+            else if (fi == 4 && fo == 1) eop = ELT_RED_4_1;
+            else if (fi == 4 && fo == 2) eop = ELT_RED_4_2;
+            else if (fi == AXHEL_HIP_RNS_ANY_INPUT) eop = ELT_RED_BARRETT;  // in_mod_factor == modulus, every limb
+            else return AXHEL_HIP_UNSUPPORTED_FACTOR;
+            break;
+        }
+        default:
+            return AXHEL_HIP_INVALID_ARGUMENT;
+    }
+    const uint64_t nl = count * L;
+    void *scratch = nullptr;
+    AXHEL_HIP_TRY(hipMallocAsync(&scratch, nl * sizeof(EltLimbDev), st));
+    EltLimbDev *d = (EltLimbDev *)scratch;
+    hipLaunchKernelGGL(rns_fill_elt, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st, d, ctx->d_info, result,
+                       operand1, b, (uint32_t)count, L, ctx->n, sc);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = launch_elt_batch(eop, f, d, (uint32_t)nl, ctx->n, st);
+    const hipError_t fe = hipFreeAsync(scratch, st);
+    AXHEL_HIP_TRY(e);
+    AXHEL_HIP_TRY(fe);
+    return AXHEL_HIP_OK;
+}
+
+}  // extern "C"

@@ -291,6 +291,20 @@ def main():
     eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     log(f"same step as eager launches: {eager_ms:.3f} ms/step")
 
+    # the same step as one RNS-context call (job tables built on the device
+    # per call, no graph): what a SEAL-style limb loop gets without a Batch
+    rns = ax.RNS(n, primes)
+    for _ in range(10):
+        rns.ntt(dst, src, P, False, 1, stream=run_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rns.ntt(dst, src, P, False, 1, stream=run_stream)
+    torch.cuda.synchronize()
+    rns_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log(f"same step as one RNS-context call: {rns_ms:.3f} ms/step")
+    rns.close()
+
     # inverse (reported beside)
     ibatch = ax.Batch([(plans[l], src[p, l], dst[p, l], True, 1) for p in range(P) for l in range(L)])
     ibatch.launch(run_stream)
@@ -429,6 +443,7 @@ def main():
         "inverse_limbs_per_s": inv_rate,
         "kernel_ms": kdur,
         "eager_ms_per_step": eager_ms,
+        "rns_context_ms_per_step": rns_ms,
         "separate_stream_graph_ms_per_step": stream_ms,
         "roofline": {"bound": "valu", "achieved": achieved_bfly, "peak": peak_bfly, "unit": "Gbfly/s",
                      "frac": achieved_bfly / peak_bfly, "traffic": traffic,

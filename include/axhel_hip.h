@@ -285,6 +285,49 @@ axhel_hip_status axhel_hip_server_stats(axhel_hip_server *server, axhel_hip_serv
 /* Clears latency samples and counters (after a warm-up). */
 axhel_hip_status axhel_hip_server_reset_stats(axhel_hip_server *server);
 
+/* ------------------------------------------------------------ RNS context */
+
+/* N and L primes with one resident plan per prime, and calls over whole RNS
+   polynomials: `count` polynomials stored contiguously as
+   [count][limbs][degree] uint64 words in device memory (limb l of
+   polynomial p at offset (p * limbs + l) * degree).  One call does what the
+   reference's caller does with one ComputeForward/ComputeInverse or Eltwise*Mod
+   call per limb (integration/seal/SEAL-4.4.5-AXHEL.patch:185-226: the limb
+   loop); the arithmetic per limb is that call's.  Calls are asynchronous on
+   `stream`; their scratch is stream-ordered, so calls may run on several
+   streams at once.  No reference counterpart. */
+typedef struct axhel_hip_rns axhel_hip_rns;
+
+#define AXHEL_HIP_RNS_MAX_LIMBS 64
+/* input_mod_factor of AXHEL_HIP_OP_REDUCE meaning "equal to each limb's
+   modulus": Barrett reduction of arbitrary 64-bit words (the in == modulus
+   branch of eltwise-reduce-mod.cpp:32-128). */
+#define AXHEL_HIP_RNS_ANY_INPUT 0xFFFFFFFFFFFFFFFFull
+
+/* roots: primitive 2N-th roots, one per modulus, or NULL for the first
+   a = 2, 3, ... whose a^((q-1)/2N) has order 2N (parameters.hpp:110-122).
+   limbs in [1, AXHEL_HIP_RNS_MAX_LIMBS]. */
+axhel_hip_status axhel_hip_rns_create(uint64_t degree, const uint64_t *moduli, const uint64_t *roots, uint32_t limbs,
+                                      axhel_hip_rns **ctx);
+axhel_hip_status axhel_hip_rns_destroy(axhel_hip_rns *ctx);
+/* The plan of limb l (owned by the context). */
+const axhel_hip_plan *axhel_hip_rns_plan(const axhel_hip_rns *ctx, uint32_t limb);
+/* Forward (inverse = 0) or inverse (1) NTT of every limb of `count`
+   polynomials; output_mod_factor as ComputeForward/ComputeInverse (lazy for
+   4 / 2, normalized otherwise).  result may equal operand. */
+axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const uint64_t *operand, uint64_t count,
+                                   int inverse, uint64_t output_mod_factor, void *stream);
+/* Element-wise op over every limb with that limb's modulus: op is
+   AXHEL_HIP_OP_ADD / SUB (operand2 required), MUL (operand2 required,
+   input_mod_factor in {1,2,4}), FMA (scalars[limbs] on the host, operand2 =
+   addend or NULL, input_mod_factor in {1,2,4,8}) or REDUCE (input/output
+   factors as EltwiseReduceMod, AXHEL_HIP_RNS_ANY_INPUT for the Barrett
+   branch).  Unsupported factors return AXHEL_HIP_UNSUPPORTED_FACTOR and leave
+   result untouched. */
+axhel_hip_status axhel_hip_rns_eltwise(axhel_hip_rns *ctx, int op, uint64_t *result, const uint64_t *operand1,
+                                       const uint64_t *operand2, const uint64_t *scalars, uint64_t count,
+                                       uint64_t input_mod_factor, uint64_t output_mod_factor, void *stream);
+
 /* ---------------------------------------------------------------- tuning */
 
 /* Kernel configuration for one degree.  A transform runs as one ROW pass

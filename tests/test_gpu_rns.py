@@ -1,0 +1,134 @@
+"""RNS context: calls over count polynomials x L limbs stored [count][L][N]
+equal the reference's per-limb calls (the caller's limb loop,
+integration/seal/SEAL-4.4.5-AXHEL.patch:185-226), limb by limb, bit for bit;
+sampled limbs equal the oracle."""
+import numpy as np
+import pytest
+
+import axhel_amd as ax
+from axhel_amd import params
+from oracle import oracle as O
+from tests.helpers import host, torch_mod
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(n, L, count, seed, bound_factor=1):
+    torch = torch_mod()
+    qs = params.rns_primes(n, L)
+    ctx = ax.RNS(n, qs)
+    rng = np.random.default_rng(seed)
+    data = np.stack([np.stack([params.uniform(rng, n, bound_factor * q) for q in qs]) for _ in range(count)])
+    src = torch.from_numpy(data.view(np.int64).copy()).cuda()
+    return torch, qs, ctx, data, src
+
+
+@pytest.mark.parametrize("logn,count", [(12, 3), (12, 40), (16, 5), (16, 24)])
+def test_rns_ntt_equals_per_limb_calls(logn, count):
+    n, L = 1 << logn, 6
+    for inverse in (False, True):
+        f_in = 2 if inverse else 4
+        torch, qs, ctx, data, src = setup(n, L, count, logn + count + inverse, f_in)
+        plans = [ax.NTT(n, q, params.find_primitive_2n_root(n, q)) for q in qs]
+        for lazy in (True, False):
+            of = f_in if lazy else 1
+            out = torch.full_like(src, -1)
+            ctx.ntt(out, src, count, inverse, of)
+            ref = torch.full_like(src, -1)
+            for p in range(count):
+                for l in range(L):
+                    (plans[l].ComputeInverse if inverse else plans[l].ComputeForward)(ref[p, l], src[p, l], 1, of)
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref), (logn, count, inverse, lazy)
+            t = O.Tables(n, qs[L - 1], params.find_primitive_2n_root(n, qs[L - 1]))
+            a = data[count - 1, L - 1]
+            want = O.compute_inverse(a, t, f_in, of) if inverse else O.compute_forward(a, t, f_in, of)
+            assert np.array_equal(host(out[count - 1, L - 1]), want)
+        # in place
+        buf = src.clone()
+        ctx.ntt(buf, buf, count, inverse, 1)
+        ctx.ntt(out, src, count, inverse, 1)
+        torch.cuda.synchronize()
+        assert torch.equal(buf, out)
+        ctx.close()
+
+
+def test_rns_default_roots_follow_the_reference_rule():
+    n = 1 << 10
+    qs = params.rns_primes(n, 3)
+    ctx = ax.RNS(n, qs)
+    for l, q in enumerate(qs):
+        plan = ax.NTT(n, q, params.find_primitive_2n_root(n, q))
+        fwd, inv, ni = plan.tables()
+        got = np.zeros(2 * n, dtype=np.uint64)
+        assert ax.lib().axhel_hip_ntt_plan_tables(ctx.plan_handle(l), got.ctypes.data, None, None) == 0
+        assert np.array_equal(got, fwd), l
+    ctx.close()
+
+
+@pytest.mark.parametrize("logn", [12, 15])
+def test_rns_eltwise_equals_per_limb_calls(logn):
+    n, L, count = 1 << logn, 5, 7
+    torch, qs, ctx, a_np, a = setup(n, L, count, 11, 4)
+    rng = np.random.default_rng(12)
+    b_np = np.stack([np.stack([params.uniform(rng, n, 4 * q) for q in qs]) for _ in range(count)])
+    b = torch.from_numpy(b_np.view(np.int64).copy()).cuda()
+    lo = torch.remainder(a, torch.tensor(qs, device="cuda").view(1, L, 1))  # canonical operands
+    lo_b = torch.remainder(b, torch.tensor(qs, device="cuda").view(1, L, 1))
+    scalars = [int(params.uniform(rng, 1, 8 * q)[0]) for q in qs]
+    cases = [
+        (ax.OP_ADD, lo, lo_b, None, 1, 1, lambda r, x, y, q, l: ax.EltwiseAddMod(r, x, y, n, q)),
+        (ax.OP_SUB, lo, lo_b, None, 1, 1, lambda r, x, y, q, l: ax.EltwiseSubMod(r, x, y, n, q)),
+        (ax.OP_MUL, lo, lo_b, None, 1, 1, lambda r, x, y, q, l: ax.EltwiseMulMod(r, x, y, n, q, 1)),
+        (ax.OP_MUL, a, b, None, 4, 1, lambda r, x, y, q, l: ax.EltwiseMulMod(r, x, y, n, q, 4)),
+        (ax.OP_FMA, a, b, scalars, 8, 1, lambda r, x, y, q, l: ax.EltwiseFMAMod(r, x, scalars[l], y, n, q, 8)),
+        (ax.OP_FMA, lo, None, scalars, 8, 1, lambda r, x, y, q, l: ax.EltwiseFMAMod(r, x, scalars[l], None, n, q, 8)),
+        (ax.OP_REDUCE, a, None, None, 4, 1, lambda r, x, y, q, l: ax.EltwiseReduceMod(r, x, n, q, 4, 1)),
+        (ax.OP_REDUCE, a, None, None, 4, 2, lambda r, x, y, q, l: ax.EltwiseReduceMod(r, x, n, q, 4, 2)),
+        (ax.OP_REDUCE, a, None, None, ax.RNS_ANY_INPUT, 1, lambda r, x, y, q, l: ax.EltwiseReduceMod(r, x, n, q, q, 1)),
+        (ax.OP_REDUCE, a, None, None, 1, 1, lambda r, x, y, q, l: ax.EltwiseReduceMod(r, x, n, q, 1, 1)),
+    ]
+    for i, (op, x, y, sc, fi, fo, plain) in enumerate(cases):
+        out = torch.full_like(a, -1)
+        ctx.eltwise(op, out, x, y, sc, count, fi, fo)
+        ref = torch.full_like(a, -1)
+        for p in range(count):
+            for l in range(L):
+                plain(ref[p, l], x[p, l], None if y is None else y[p, l], qs[l], l)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), (logn, i)
+    # one oracle sample: MulMod F=1 on the last limb
+    out = torch.empty_like(a)
+    ctx.eltwise(ax.OP_MUL, out, lo, lo_b, None, count, 1, 1)
+    want = O.mul_mod(host(lo[count - 1, L - 1]), host(lo_b[count - 1, L - 1]), qs[L - 1], 1)
+    assert np.array_equal(host(out[count - 1, L - 1]), want)
+    # unsupported factors leave the result untouched
+    keep = torch.full_like(a, 7)
+    with pytest.raises(ax.AxhelError):
+        ctx.eltwise(ax.OP_MUL, keep, lo, lo_b, None, count, 3, 1)
+    with pytest.raises(ax.AxhelError):
+        ctx.eltwise(ax.OP_REDUCE, keep, a, None, None, count, 8, 1)
+    torch.cuda.synchronize()
+    assert bool((keep == 7).all())
+    ctx.close()
+
+
+def test_rns_calls_on_two_streams():
+    """Scratch is stream-ordered: calls issued on two streams at once do not
+    share job tables."""
+    n, L, count = 1 << 14, 4, 16
+    torch, qs, ctx, data, src = setup(n, L, count, 21, 4)
+    src2 = torch.flip(src, dims=[0]).contiguous()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    o1, o2 = torch.empty_like(src), torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        ctx.ntt(o1, src, count, False, 1, stream=s1)
+        ctx.ntt(o2, src2, count, False, 1, stream=s2)
+    torch.cuda.synchronize()
+    r1 = torch.empty_like(src)
+    ctx.ntt(r1, src, count, False, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(o1, r1)
+    assert torch.equal(o2, torch.flip(r1, dims=[0]))
+    ctx.close()
