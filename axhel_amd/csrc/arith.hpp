@@ -117,9 +117,9 @@ __device__ __forceinline__ uint64_t twice_plus(uint64_t a, uint64_t b) {
     return r;
 }
 
-// Per-limb constants of the butterflies: q, 2q and their negations mod 2^64.
+// Per-limb constants of the butterflies: q, 2q, 4q and their negations mod 2^64.
 struct ModConsts {
-    uint64_t q, q2, nq, nq2;
+    uint64_t q, q2, nq, nq2, q4, nq4;
 };
 
 __device__ __forceinline__ ModConsts mod_consts(uint64_t q) {
@@ -128,6 +128,8 @@ __device__ __forceinline__ ModConsts mod_consts(uint64_t q) {
     m.q2 = q << 1;
     m.nq = 0 - q;
     m.nq2 = 0 - m.q2;
+    m.q4 = q << 2;
+    m.nq4 = 0 - m.q4;
     return m;
 }
 
@@ -156,6 +158,32 @@ __device__ __forceinline__ uint64_t csub_n(uint64_t x, uint64_t b, uint64_t nb) 
 // gives the reference's values exactly.
 __device__ __forceinline__ void fwd_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wq, const ModConsts &m) {
     const uint64_t xh = csub_n(x, m.q2, m.nq2);
+    const uint64_t sum = shoup_acc(xh, y, w, wq, m.nq);
+    y = sub64(twice_plus(xh, m.q2), sum);
+    x = sum;
+}
+
+// Forward butterfly of the relaxed schedule, for transforms whose output is
+// normalized to [0, q) and whose modulus is below 2^61.  A normalized output
+// is the unique canonical residue, so any congruent intermediate values give
+// the reference's result bit for bit (for inputs in its domain [0, 4q),
+// ntt-native.cpp:17); the schedule trades the per-stage bound for fewer
+// conditional subtractions:
+//   SKIP (odd local stage):  x' = x            x < 6q  -> outputs < 8q
+//   SUB4 (even local stage): x' = CS(x, 4q)    x < 8q  -> x' < 4q, outputs < 6q
+//   LAST (bit 0, last pass): x' = CS(CS(x,4q),2q)       -> outputs < 4q
+// Every stage reads x' + t and x' + 2q - t with t = Shoup(y) in [0, 2q) for
+// any 64-bit y, and no two SKIP stages are adjacent (bit 0 of every pass is
+// even), so every word stays below 8q < 2^64.  LAST leaves the [0, 4q) range
+// the normalization of the exact schedule expects.
+enum class FwdStage { SKIP, SUB4, LAST };
+
+template <FwdStage S>
+__device__ __forceinline__ void fwd_bfly_relaxed(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wq,
+                                                 const ModConsts &m) {
+    uint64_t xh = x;
+    if constexpr (S != FwdStage::SKIP) xh = csub_n(xh, m.q4, m.nq4);
+    if constexpr (S == FwdStage::LAST) xh = csub_n(xh, m.q2, m.nq2);
     const uint64_t sum = shoup_acc(xh, y, w, wq, m.nq);
     y = sub64(twice_plus(xh, m.q2), sum);
     x = sum;

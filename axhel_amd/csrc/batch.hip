@@ -29,6 +29,7 @@ struct axhel_hip_batch {
         uint32_t count;
         size_t offset;  // first descriptor in d_jobs
         bool vec16;     // all src/dst 16-byte aligned
+        bool relax_ok;  // every modulus below 2^61 (relaxed forward schedule allowed)
         // stacked ROW pass: stack table at d_jobs + stack_offset (stack_gpw 0: none)
         size_t stack_offset;
         uint32_t stacks;
@@ -71,7 +72,7 @@ hipError_t launch_group(const axhel_hip_batch *b, const axhel_hip_batch::Group &
     static const NttJobDev none{};
     const StackTable st = stack_of(b, g);
     return launch_ntt(config_for(g.logn), g.logn, g.inverse, g.normalize, b->d_jobs + g.offset, none, g.count,
-                      stream, only_pass, g.vec16, &st);
+                      stream, only_pass, g.vec16, &st, g.relax_ok);
 }
 
 hipError_t issue(axhel_hip_batch *b, hipStream_t stream) {
@@ -166,9 +167,11 @@ axhel_hip_status axhel_hip_ntt_batch_create(const axhel_hip_ntt_job *jobs, size_
         g.count = (uint32_t)idx.size();
         g.offset = host.size();
         g.vec16 = true;
+        g.relax_ok = true;
         for (size_t i : idx) {
             host.push_back(make_job(jobs[i].plan, jobs[i].operand, jobs[i].result, g.inverse));
             g.vec16 = g.vec16 && aligned16(jobs[i].operand) && aligned16(jobs[i].result);
+            g.relax_ok = g.relax_ok && relax_ok_modulus(jobs[i].plan->q);
         }
         g.stack_offset = 0;
         g.stacks = 0;

@@ -109,6 +109,8 @@ template <int LG, int LE, int GPW, bool COL, bool MULTI, bool FWD>
 PassKernel pk() {
     PassKernel k;
     k.fn = reinterpret_cast<const void *>(&axhel_dev::ntt_pass_kernel<LG, LE, GPW, FWD, COL, MULTI>);
+    if constexpr (FWD)
+        k.fn_relaxed = reinterpret_cast<const void *>(&axhel_dev::ntt_pass_kernel<LG, LE, GPW, true, COL, MULTI, true>);
     k.log_g = LG;
     k.log_e = LE;
     k.gpw = GPW;
@@ -152,6 +154,8 @@ template <int LG, int LE, int GPW, bool FWD>
 PassKernel pk_stack() {
     PassKernel k;
     k.fn = reinterpret_cast<const void *>(&axhel_dev::ntt_row_stack_kernel<LG, LE, GPW, FWD>);
+    if constexpr (FWD)
+        k.fn_relaxed = reinterpret_cast<const void *>(&axhel_dev::ntt_row_stack_kernel<LG, LE, GPW, true, true>);
     k.log_g = LG;
     k.log_e = LE;
     k.gpw = GPW;
@@ -276,16 +280,21 @@ const std::vector<std::vector<NttConfig>> &registry() {
 std::mutex g_attr_mu;
 std::vector<std::pair<const void *, int>> g_attr_done;
 
+hipError_t ensure_fn_lds(const void *fn, size_t bytes, int dev) {
+    std::lock_guard<std::mutex> lk(g_attr_mu);
+    for (auto &d : g_attr_done)
+        if (d.first == fn && d.second == dev) return hipSuccess;
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) g_attr_done.emplace_back(fn, dev);
+    return e;
+}
+
 hipError_t ensure_lds(const PassKernel &k) {
     if (k.lds_bytes <= 65536) return hipSuccess;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return e;
-    std::lock_guard<std::mutex> lk(g_attr_mu);
-    for (auto &d : g_attr_done)
-        if (d.first == k.fn && d.second == dev) return hipSuccess;
-    e = hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.lds_bytes);
-    if (e == hipSuccess) g_attr_done.emplace_back(k.fn, dev);
+    if (e == hipSuccess) e = ensure_fn_lds(k.fn, k.lds_bytes, dev);
+    if (e == hipSuccess && k.fn_relaxed) e = ensure_fn_lds(k.fn_relaxed, k.lds_bytes, dev);
     return e;
 }
 
@@ -368,9 +377,14 @@ NttJobDev make_job(const axhel_hip_plan *p, const uint64_t *src, uint64_t *dst, 
 
 hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool normalize, const NttJobDev *jobs,
                       const NttJobDev &inline_job, uint32_t count, hipStream_t stream, int only_pass, bool vec16,
-                      const StackTable *stacks) {
+                      const StackTable *stacks, bool relax_ok) {
     if (count == 0) return hipSuccess;
     const uint64_t n = 1ull << logn;
+    // every pass of the transform on one schedule: relaxed only when each
+    // forward kernel that may run has its relaxed twin
+    bool relaxed = !inverse && normalize && relax_ok;
+    for (int p = 0; p < cfg.npass && relaxed; ++p) relaxed = cfg.fwd[p].fn_relaxed != nullptr;
+    if (relaxed && cfg.fwd_stack.fn && !cfg.fwd_stack.fn_relaxed) relaxed = false;
     for (int p = 0; p < cfg.npass; ++p) {
         if (only_pass >= 0 && p != only_pass) continue;
         const bool row_pass = cfg.npass == 2 && p == (inverse ? 0 : 1);
@@ -403,7 +417,8 @@ hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool no
         // the kernels index groups with 32-bit arithmetic (tile * GPW + g)
         if (blocks == 0 || blocks * (uint64_t)k.gpw > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
         void *args[] = {&a};
-        e = hipLaunchKernel(k.fn, dim3((unsigned)blocks), dim3(k.threads), args, k.lds_bytes, stream);
+        e = hipLaunchKernel(relaxed ? k.fn_relaxed : k.fn, dim3((unsigned)blocks), dim3(k.threads), args,
+                            k.lds_bytes, stream);
         if (e != hipSuccess) return e;
     }
     return hipGetLastError();
@@ -474,7 +489,8 @@ axhel_hip_status run_one(const axhel_hip_plan *plan, uint64_t *result, const uin
     AXHEL_HIP_TRY(stg.output(result, plan->n, &dres));
     const NttJobDev job = make_job(plan, dop, dres, inverse);
     const bool vec16 = aligned16(dop) && aligned16(dres);
-    AXHEL_HIP_TRY(launch_ntt(config_for(plan->logn), plan->logn, inverse, normalize, nullptr, job, 1, st, -1, vec16));
+    AXHEL_HIP_TRY(launch_ntt(config_for(plan->logn), plan->logn, inverse, normalize, nullptr, job, 1, st, -1, vec16,
+                             nullptr, relax_ok_modulus(plan->q)));
     AXHEL_HIP_TRY(stg.finish());
     return AXHEL_HIP_OK;
 }

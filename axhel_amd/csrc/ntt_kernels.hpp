@@ -15,7 +15,11 @@
 // with the reference's twiddle, and butterflies on one bit run only after all
 // butterflies on the previous bit that feed them, so outputs are bit-identical
 // to the scalar loop, lazy bits included.  Only the order of independent
-// butterflies changes.
+// butterflies changes.  The one exception is opt-in (RLX = true, forward
+// kernels launched for a normalized output with every q < 2^61): the
+// conditional subtractions follow the relaxed schedule of fwd_bfly_relaxed,
+// whose intermediate words differ but whose normalized outputs, the unique
+// residues in [0, q), are the reference's.
 //
 // Mapping to the hardware.  A pass handles a contiguous range of LOG_G bit
 // positions.  Fixing every other index bit leaves a "group" of G = 2^LOG_G
@@ -307,7 +311,10 @@ struct TwStage {
 // The butterfly chunks of one pass over registers v (chunk-0 layout on entry,
 // last-chunk layout on exit), redistributing through LDS between chunks.
 // Every thread of the workgroup must call it (it holds barriers).
-template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, typename TW>
+// RLX: forward butterflies on the relaxed schedule of fwd_bfly_relaxed (local
+// bit p odd: SKIP; even: SUB4; bit 0 of a ROW or single pass, the transform's
+// last stage: LAST).
+template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, bool RLX = false, typename TW>
 __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *lds, uint32_t g, uint32_t tid,
                                            const ModConsts &m, const TW &tws, bool final_inv, const NttJobDev &J) {
     using Gm = Geom<LOG_G, LOG_E, FWD>;
@@ -356,7 +363,12 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
                             wq[r] = tv.y;
                         }
                     }
-                    if constexpr (FWD) {
+                    if constexpr (FWD && RLX) {
+                        constexpr FwdStage S = (p & 1)             ? FwdStage::SKIP
+                                               : (!COL && p == 0) ? FwdStage::LAST
+                                                                  : FwdStage::SUB4;
+                        fwd_bfly_relaxed<S>(v[r], v[r | bit], w[rp], wq[rp], m);
+                    } else if constexpr (FWD) {
                         fwd_bfly(v[r], v[r | bit], w[rp], wq[rp], m);
                     } else {
                         if (fin)
@@ -370,7 +382,7 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
     });
 }
 
-template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, bool MULTI>
+template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, bool MULTI, bool RLX = false>
 __global__ void __launch_bounds__(GPW << (LOG_G - LOG_E))
 ntt_pass_kernel(PassArgs a) {
     using Gm = Geom<LOG_G, LOG_E, FWD>;
@@ -504,10 +516,10 @@ ntt_pass_kernel(PassArgs a) {
         tstage.store(tw_lds, t);
         __syncthreads();
         const TwLds<LOG_G, FWD> tws{tw_lds};
-        run_chunks<LOG_G, LOG_E, GPW, FWD, COL>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
+        run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
     } else {
         const TwGlobal<LOG_G, FWD, COL> tws{tw, n, blo, o};
-        run_chunks<LOG_G, LOG_E, GPW, FWD, COL>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
+        run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
     }
 
     // ---- normalisation folded into the last pass (ntt.cpp:79-118)
@@ -583,7 +595,7 @@ ntt_pass_kernel(PassArgs a) {
 // twiddle pairs, which are staged in LDS once (for the per-limb ROW pass
 // each row needs its own set, twice the row's own bytes, from L2).  Bits,
 // butterflies and twiddles are those of ntt_pass_kernel's ROW pass.
-template <int LOG_G, int LOG_E, int GPW, bool FWD>
+template <int LOG_G, int LOG_E, int GPW, bool FWD, bool RLX = false>
 __global__ void __launch_bounds__(GPW << (LOG_G - LOG_E))
 ntt_row_stack_kernel(PassArgs a) {
     using Gm = Geom<LOG_G, LOG_E, FWD>;
@@ -661,7 +673,7 @@ ntt_row_stack_kernel(PassArgs a) {
     }
 
     const TwLds<LOG_G, FWD> tws{tw_lds};
-    run_chunks<LOG_G, LOG_E, GPW, FWD, false>(v, lds, g, tid, m, tws, false, J0);
+    run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX>(v, lds, g, tid, m, tws, false, J0);
 
     if (a.normalize) {
         static_for<0, E>([&](auto rc) {

@@ -40,6 +40,7 @@ struct axhel_hip_rns {
     uint32_t limbs = 0;
     std::vector<axhel_hip_plan *> plans;
     RnsLimbDev *d_info = nullptr;
+    bool relax_ok = true;  // every modulus below 2^61
 };
 
 namespace {
@@ -158,6 +159,7 @@ axhel_hip_status axhel_hip_rns_create(uint64_t degree, const uint64_t *moduli, c
         if (st != AXHEL_HIP_OK) break;
         c->plans.push_back(p);
         c->logn = p->logn;
+        c->relax_ok = c->relax_ok && relax_ok_modulus(p->q);
         info[l] = RnsLimbDev{p->d_fwd, p->d_inv, p->q, p->ninv, p->ninv_q, p->sroot, p->sroot_q, barrett_for(p->q)};
     }
     if (st == AXHEL_HIP_OK) {
@@ -228,7 +230,7 @@ axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const u
         }
         const bool vec16 = aligned16(result) && aligned16(operand);
         e = launch_ntt(cfg, ctx->logn, inverse != 0, normalize, jobs, none, (uint32_t)nj, st, -1, vec16,
-                       stacks ? &tab : nullptr);
+                       stacks ? &tab : nullptr, ctx->relax_ok);
     }
     const hipError_t fe = hipFreeAsync(scratch, st);
     AXHEL_HIP_TRY(e);

@@ -267,9 +267,11 @@ hipError_t launch_batch(axhel_hip_server *s, Slot &sl) {
         const uint32_t logn = std::get<0>(kv.first);
         const NttConfig &cfg = config_for(logn);
         if ((e = prepare_config(cfg)) != hipSuccess) return e;
+        bool relax_ok = true;
+        for (const NttJobDev &j : kv.second) relax_ok = relax_ok && relax_ok_modulus(j.q);
         e = launch_ntt(cfg, logn, std::get<1>(kv.first) != 0, std::get<2>(kv.first) != 0,
                        (const NttJobDev *)places[pi].first, none, (uint32_t)places[pi].second, st, -1,
-                       std::get<3>(kv.first) != 0);
+                       std::get<3>(kv.first) != 0, nullptr, relax_ok);
         if (e != hipSuccess) return e;
         ++pi;
     }

@@ -180,3 +180,55 @@ def test_device_generated_tables_equal_the_oracle(logn):
             assert np.array_equal(fwd, t.fwd), (logn, bits)
             assert np.array_equal(inv, t.inv), (logn, bits)
             assert np.array_equal(ni, t.ninv), (logn, bits)
+
+
+@pytest.mark.parametrize("logn", [1, 4, 12, 13, 14, 16, 17])
+def test_relaxed_schedule_boundary_moduli(logn):
+    """A normalized forward with every modulus below 2^61 runs on the relaxed
+    butterfly schedule (intermediate words up to 8q); at or above 2^61 it runs
+    the reference's schedule.  The largest NTT prime below 2^61 with inputs at
+    the top of the domain (every word 4q - 1) and random inputs, and the
+    largest below 2^62, equal the oracle through plain calls, a batch group
+    that mixes both (exact schedule for the group), a batch group of the
+    61-bit prime alone (relaxed) and an RNS context over both."""
+    torch = pytest.importorskip("torch")
+    n = 1 << logn
+    qs = [params.find_ntt_prime(n, 61), params.find_ntt_prime(n, 62)]
+    assert qs[0] < 1 << 61 <= qs[1]
+    psis = [params.find_primitive_2n_root(n, q) for q in qs]
+    plans = [ax.NTT(n, q, p) for q, p in zip(qs, psis)]
+    tabs = [O.Tables(n, q, p) for q, p in zip(qs, psis)]
+    ins = []
+    for l, q in enumerate(qs):
+        ins.append([np.full(n, 4 * q - 1, np.uint64), O.MT19937_64(SEED + l).random_vector(n, 4 * q)])
+    want = [[O.compute_forward(a, tabs[l], 1, 1) for a in ins[l]] for l in range(2)]
+    for l in range(2):
+        for k, a in enumerate(ins[l]):
+            r = empty_dev(n)
+            plans[l].ComputeForward(r, dev(a), 1, 1)
+            assert np.array_equal(host(r), want[l][k]), ("plain", l, k)
+    src = torch.from_numpy(np.stack([np.stack(ins[0]), np.stack(ins[1])]).view(np.int64)).cuda()  # [limb][k][n]
+    out = torch.full_like(src, -1)
+    mixed = ax.Batch([(plans[l], out[l, k], src[l, k], False, 1) for l in range(2) for k in range(2)])
+    mixed.launch()
+    torch.cuda.synchronize()
+    for l in range(2):
+        for k in range(2):
+            assert np.array_equal(host(out[l, k]), want[l][k]), ("mixed batch", l, k)
+    out.fill_(-1)
+    alone = ax.Batch([(plans[0], out[0, k], src[0, k], False, 1) for k in range(2)])
+    alone.launch()
+    torch.cuda.synchronize()
+    for k in range(2):
+        assert np.array_equal(host(out[0, k]), want[0][k]), ("61-bit batch", k)
+    mixed.close()
+    alone.close()
+    ctx = ax.RNS(n, qs, psis)
+    poly = src.permute(1, 0, 2).contiguous()  # [k][limb][n]
+    res = torch.full_like(poly, -1)
+    ctx.ntt(res, poly, 2, False, 1)
+    torch.cuda.synchronize()
+    for l in range(2):
+        for k in range(2):
+            assert np.array_equal(host(res[k, l]), want[l][k]), ("rns", l, k)
+    ctx.close()
