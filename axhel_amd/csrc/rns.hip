@@ -252,11 +252,17 @@ axhel_hip_status axhel_hip_rns_eltwise(axhel_hip_rns *ctx, int op, uint64_t *res
     int eop = 0, f = 1;
     const uint64_t *b = nullptr;
     switch (op) {
-        case AXHEL_HIP_OP_ADD:
+        case AXHEL_HIP_OP_ADD:  // eltwise-{add,sub}-mod.cpp: vv with operand2, else vs with scalars[l]
         case AXHEL_HIP_OP_SUB:
-            if (!operand2) return AXHEL_HIP_INVALID_ARGUMENT;
-            eop = op == AXHEL_HIP_OP_ADD ? ELT_ADD_VV : ELT_SUB_VV;
-            b = operand2;
+            if (operand2) {
+                eop = op == AXHEL_HIP_OP_ADD ? ELT_ADD_VV : ELT_SUB_VV;
+                b = operand2;
+            } else if (scalars) {
+                eop = op == AXHEL_HIP_OP_ADD ? ELT_ADD_VS : ELT_SUB_VS;
+                for (uint32_t l = 0; l < L; ++l) sc.s[l] = scalars[l];
+            } else {
+                return AXHEL_HIP_INVALID_ARGUMENT;
+            }
             break;
         case AXHEL_HIP_OP_MUL:  // eltwise-mul-mod.cpp:81-94: F in {1, 2, 4}
             if (!operand2) return AXHEL_HIP_INVALID_ARGUMENT;

@@ -318,7 +318,9 @@ const axhel_hip_plan *axhel_hip_rns_plan(const axhel_hip_rns *ctx, uint32_t limb
 axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const uint64_t *operand, uint64_t count,
                                    int inverse, uint64_t output_mod_factor, void *stream);
 /* Element-wise op over every limb with that limb's modulus: op is
-   AXHEL_HIP_OP_ADD / SUB (operand2 required), MUL (operand2 required,
+   AXHEL_HIP_OP_ADD / SUB (operand2, or NULL and scalars[limbs] on the host
+   for the vector-scalar forms, eltwise-add-mod.hpp:17 and
+   eltwise-sub-mod.hpp:17), MUL (operand2 required,
    input_mod_factor in {1,2,4}), FMA (scalars[limbs] on the host, operand2 =
    addend or NULL, input_mod_factor in {1,2,4,8}) or REDUCE (input/output
    factors as EltwiseReduceMod, AXHEL_HIP_RNS_ANY_INPUT for the Barrett

@@ -76,9 +76,12 @@ def test_rns_eltwise_equals_per_limb_calls(logn):
     lo = torch.remainder(a, torch.tensor(qs, device="cuda").view(1, L, 1))  # canonical operands
     lo_b = torch.remainder(b, torch.tensor(qs, device="cuda").view(1, L, 1))
     scalars = [int(params.uniform(rng, 1, 8 * q)[0]) for q in qs]
+    small = [int(params.uniform(rng, 1, q)[0]) for q in qs]  # vector-scalar add/sub take a scalar below q
     cases = [
         (ax.OP_ADD, lo, lo_b, None, 1, 1, lambda r, x, y, q, l: ax.EltwiseAddMod(r, x, y, n, q)),
         (ax.OP_SUB, lo, lo_b, None, 1, 1, lambda r, x, y, q, l: ax.EltwiseSubMod(r, x, y, n, q)),
+        (ax.OP_ADD, lo, None, small, 1, 1, lambda r, x, y, q, l: ax.EltwiseAddMod(r, x, small[l], n, q)),
+        (ax.OP_SUB, lo, None, small, 1, 1, lambda r, x, y, q, l: ax.EltwiseSubMod(r, x, small[l], n, q)),
         (ax.OP_MUL, lo, lo_b, None, 1, 1, lambda r, x, y, q, l: ax.EltwiseMulMod(r, x, y, n, q, 1)),
         (ax.OP_MUL, a, b, None, 4, 1, lambda r, x, y, q, l: ax.EltwiseMulMod(r, x, y, n, q, 4)),
         (ax.OP_FMA, a, b, scalars, 8, 1, lambda r, x, y, q, l: ax.EltwiseFMAMod(r, x, scalars[l], y, n, q, 8)),
@@ -102,6 +105,13 @@ def test_rns_eltwise_equals_per_limb_calls(logn):
     ctx.eltwise(ax.OP_MUL, out, lo, lo_b, None, count, 1, 1)
     want = O.mul_mod(host(lo[count - 1, L - 1]), host(lo_b[count - 1, L - 1]), qs[L - 1], 1)
     assert np.array_equal(host(out[count - 1, L - 1]), want)
+    # vector-scalar add against the oracle on one limb
+    out = torch.empty_like(a)
+    ctx.eltwise(ax.OP_ADD, out, lo, None, small, count, 1, 1)
+    want = O.add_mod(host(lo[0, 1]), small[1], qs[1])
+    assert np.array_equal(host(out[0, 1]), want)
+    with pytest.raises(ax.AxhelError):  # add/sub need operand2 or scalars
+        ctx.eltwise(ax.OP_ADD, out, lo, None, None, count, 1, 1)
     # unsupported factors leave the result untouched
     keep = torch.full_like(a, 7)
     with pytest.raises(ax.AxhelError):
