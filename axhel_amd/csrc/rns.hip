@@ -141,6 +141,8 @@ axhel_hip_status axhel_hip_rns_create(uint64_t degree, const uint64_t *moduli, c
                                       axhel_hip_rns **ctx) {
     if (!ctx || !moduli || limbs == 0 || limbs > kMaxLimbs) return AXHEL_HIP_INVALID_ARGUMENT;
     *ctx = nullptr;
+    // the degree rule of plan creation, checked before the root search divides by 2N
+    if (degree < 2 || (degree & (degree - 1)) != 0 || degree > (1ull << kMaxLogN)) return AXHEL_HIP_INVALID_ARGUMENT;
     axhel_hip_rns *c = new (std::nothrow) axhel_hip_rns;
     if (!c) return AXHEL_HIP_NO_MEMORY;
     c->n = degree;

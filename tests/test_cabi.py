@@ -4,6 +4,8 @@ import os
 import re
 import subprocess
 
+import pytest
+
 import axhel_amd
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -72,3 +74,18 @@ def test_configs_are_listed_without_a_gpu():
         assert c["log_elems"] in (1, 2, 3, 4)
     assert axhel_amd.config_count(3) == 0
     assert axhel_amd.config_count(1 << 18) == 0
+
+
+def test_rns_create_rejects_bad_arguments():
+    """Invalid degrees are refused before any root search (degree 0 would
+    divide by zero there); a modulus without a 2N-th root and an empty
+    modulus list are refused.  Every case returns before any HIP call."""
+    from axhel_amd import params
+    q = params.rns_primes(1 << 10, 1)[0]
+    for n in (0, 1, 3, 1000, 1 << 18):
+        with pytest.raises(axhel_amd.AxhelError):
+            axhel_amd.RNS(n, [q])
+    with pytest.raises(axhel_amd.AxhelError):
+        axhel_amd.RNS(1 << 10, [97])  # 97 - 1 is not a multiple of 2048
+    with pytest.raises(axhel_amd.AxhelError):
+        axhel_amd.RNS(1 << 10, [])

@@ -142,3 +142,4 @@ def test_rns_calls_on_two_streams():
     assert torch.equal(o1, r1)
     assert torch.equal(o2, torch.flip(r1, dims=[0]))
     ctx.close()
+
