@@ -1,7 +1,8 @@
 """The benchmark workloads at full size, checked through size-independent
 properties (the oracle is too slow for every limb at these sizes):
 
-* 64 polys x 24 limbs x N=2^16 (1536 limb-transforms, the headline batch):
+* 64 polys x 24 limbs x N=2^16 (1536 limb-transforms, the headline batch),
+  as a batch and as RNS-context calls (which must agree word for word):
   forward then inverse is the identity on every limb; the lazy forward
   reduces to the normalized forward mod q and stays below 4q; the transform
   is linear; sampled limbs equal the oracle.
@@ -62,6 +63,19 @@ def test_headline_batch_full_size_properties():
         assert np.array_equal(host(lazy[p, l]), O.compute_forward(a, t, 1, 4)), (p, l)
     for b in (bf, bl, bi, bs):
         b.close()
+    # the same transforms as RNS-context calls over the [P][L][N] array
+    ctx = ax.RNS(n, primes, psis)
+    got = torch.full_like(src, -1)
+    ctx.ntt(got, src, P, False, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(got, fwd)
+    ctx.ntt(got, src, P, False, 4)
+    torch.cuda.synchronize()
+    assert torch.equal(got, lazy)
+    ctx.ntt(got, fwd, P, True, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(got, src)
+    ctx.close()
 
 
 def test_uneven_512_jobs_full_size_equal_plain_calls():
