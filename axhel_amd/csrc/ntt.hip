@@ -125,6 +125,7 @@ template <int LG, int LE, int GPW>
 PassKernel pk_fused() {
     PassKernel k;
     k.fn = reinterpret_cast<const void *>(&axhel_dev::polymul_row_kernel<LG, LE, GPW>);
+    k.fn_relaxed = reinterpret_cast<const void *>(&axhel_dev::polymul_row_kernel<LG, LE, GPW, true>);
     k.log_g = LG;
     k.log_e = LE;
     k.gpw = GPW;
@@ -139,6 +140,7 @@ template <int LG, int LE, int GPW>
 PassKernel pk_fused_stack() {
     PassKernel k;
     k.fn = reinterpret_cast<const void *>(&axhel_dev::polymul_row_stack_kernel<LG, LE, GPW>);
+    k.fn_relaxed = reinterpret_cast<const void *>(&axhel_dev::polymul_row_stack_kernel<LG, LE, GPW, true>);
     k.log_g = LG;
     k.log_e = LE;
     k.gpw = GPW;
@@ -375,16 +377,21 @@ NttJobDev make_job(const axhel_hip_plan *p, const uint64_t *src, uint64_t *dst, 
     return j;
 }
 
+bool forward_relaxed(const NttConfig &cfg, bool normalize, bool relax_ok) {
+    // every pass of the transform on one schedule: relaxed only when each
+    // forward kernel that may run has its relaxed twin
+    bool relaxed = normalize && relax_ok;
+    for (int p = 0; p < cfg.npass && relaxed; ++p) relaxed = cfg.fwd[p].fn_relaxed != nullptr;
+    if (relaxed && cfg.fwd_stack.fn && !cfg.fwd_stack.fn_relaxed) relaxed = false;
+    return relaxed;
+}
+
 hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool normalize, const NttJobDev *jobs,
                       const NttJobDev &inline_job, uint32_t count, hipStream_t stream, int only_pass, bool vec16,
                       const StackTable *stacks, bool relax_ok) {
     if (count == 0) return hipSuccess;
     const uint64_t n = 1ull << logn;
-    // every pass of the transform on one schedule: relaxed only when each
-    // forward kernel that may run has its relaxed twin
-    bool relaxed = !inverse && normalize && relax_ok;
-    for (int p = 0; p < cfg.npass && relaxed; ++p) relaxed = cfg.fwd[p].fn_relaxed != nullptr;
-    if (relaxed && cfg.fwd_stack.fn && !cfg.fwd_stack.fn_relaxed) relaxed = false;
+    const bool relaxed = !inverse && forward_relaxed(cfg, normalize, relax_ok);
     for (int p = 0; p < cfg.npass; ++p) {
         if (only_pass >= 0 && p != only_pass) continue;
         const bool row_pass = cfg.npass == 2 && p == (inverse ? 0 : 1);

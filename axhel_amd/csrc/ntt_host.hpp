@@ -102,6 +102,12 @@ hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool no
 // (its words stay below 8q).
 inline bool relax_ok_modulus(uint64_t q) { return q < (1ull << 61); }
 
+// Whether a forward transform under cfg runs the relaxed schedule: its output
+// is normalized, every modulus passes relax_ok_modulus (relax_ok), and every
+// forward pass kernel of cfg has a relaxed twin.  launch_ntt decides by this,
+// and so must any caller that runs part of a forward transform itself.
+bool forward_relaxed(const NttConfig &cfg, bool normalize, bool relax_ok);
+
 // Group count per stack of the stacked ROW pass for log2(degree) under the
 // selected configuration (0: no stacked pass).
 int stack_gpw(uint32_t logn);

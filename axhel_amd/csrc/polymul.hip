@@ -30,6 +30,7 @@ struct axhel_hip_polymul {
         size_t off_elt_mul, off_elt_fma;  // count EltLimbDev each, unfused path
         bool vec16;
         bool has_addend;
+        bool relax_ok;  // every modulus below 2^61: the forward transforms may run relaxed
         // fused path over stacks of same-plan jobs: stack table at off_fstack
         // (fstack_gpw 0: none)
         size_t off_fstack;
@@ -92,9 +93,17 @@ hipError_t issue(axhel_hip_polymul *pm, hipStream_t st) {
         const NttJobDev *col = (const NttJobDev *)(pm->d_desc + g.off_col);
         const NttJobDev *inv = (const NttJobDev *)(pm->d_desc + g.off_inv);
         if (pm->fused && cfg.fused.fn) {
+            const bool stacked = g.fstack_gpw && cfg.fused_stack.fn && cfg.fused_stack.gpw == g.fstack_gpw;
+            const PassKernel &fk = stacked ? cfg.fused_stack : cfg.fused;
+            // the forward COL pass and the fused kernel's forward stages on one
+            // schedule (the fused kernel normalizes NTT(a) and NTT(b))
+            const bool rlx = forward_relaxed(cfg, true, g.relax_ok) && fk.fn_relaxed;
             if (cfg.npass == 2) {
-                // forward COL pass of every a and b
-                if ((e = launch_ntt(cfg, g.logn, false, false, col, none, 2 * g.count, st, 0, g.vec16))) return e;
+                // forward COL pass of every a and b (normalize applies to the last pass only, so
+                // this pass is the same; it selects the schedule)
+                if ((e = launch_ntt(cfg, g.logn, false, true, col, none, 2 * g.count, st, 0, g.vec16, nullptr,
+                                    rlx)))
+                    return e;
             }
             PolyArgs a;
             std::memset(&a, 0, sizeof(a));
@@ -104,8 +113,6 @@ hipError_t issue(axhel_hip_polymul *pm, hipStream_t st) {
             a.groups_per_limb = (uint32_t)((1ull << g.logn) >> cfg.fused.log_g);
             a.single = cfg.npass == 1;
             a.vec16 = g.vec16;
-            const bool stacked = g.fstack_gpw && cfg.fused_stack.fn && cfg.fused_stack.gpw == g.fstack_gpw;
-            const PassKernel &fk = stacked ? cfg.fused_stack : cfg.fused;
             uint64_t blocks = (uint64_t)g.count * a.groups_per_limb / cfg.fused.gpw;
             if (stacked) {
                 a.jobs = (const PolyJobDev *)(pm->d_desc + g.off_fstack);
@@ -114,14 +121,17 @@ hipError_t issue(axhel_hip_polymul *pm, hipStream_t st) {
             }
             if (blocks == 0 || blocks * (uint64_t)fk.gpw > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
             void *args[] = {&a};
-            if ((e = hipLaunchKernel(fk.fn, dim3((unsigned)blocks), dim3(fk.threads), args, fk.lds_bytes, st)))
+            if ((e = hipLaunchKernel(rlx ? fk.fn_relaxed : fk.fn, dim3((unsigned)blocks), dim3(fk.threads), args,
+                                     fk.lds_bytes, st)))
                 return e;
             if (cfg.npass == 2) {
                 if ((e = launch_ntt(cfg, g.logn, true, true, inv, none, g.count, st, 1, g.vec16))) return e;
             }
         } else {
             // ComputeForward(a) -> result, ComputeForward(b) -> scratch: one launch per pass for both
-            if ((e = launch_ntt(cfg, g.logn, false, true, col, none, 2 * g.count, st, -1, g.vec16))) return e;
+            if ((e = launch_ntt(cfg, g.logn, false, true, col, none, 2 * g.count, st, -1, g.vec16, nullptr,
+                                g.relax_ok)))
+                return e;
             const uint64_t n = 1ull << g.logn;
             if ((e = launch_elt_batch(ELT_MUL, 1, (const EltLimbDev *)(pm->d_desc + g.off_elt_mul), g.count, n, st)))
                 return e;
@@ -167,6 +177,7 @@ axhel_hip_status axhel_hip_polymul_create(const axhel_hip_polymul_job *jobs, siz
         g.count = (uint32_t)idx.size();
         g.has_addend = kv.first.second != 0;
         g.vec16 = true;
+        g.relax_ok = true;
         std::vector<NttJobDev> col, inv;
         std::vector<PolyJobDev> fz;
         std::vector<EltLimbDev> mul, fma;
@@ -175,6 +186,7 @@ axhel_hip_status axhel_hip_polymul_create(const axhel_hip_polymul_job *jobs, siz
             const axhel_hip_plan *p = j.plan;
             g.vec16 = g.vec16 && aligned16(j.a) && aligned16(j.b) && aligned16(j.result) && aligned16(j.scratch) &&
                       (!j.addend || aligned16(j.addend));
+            g.relax_ok = g.relax_ok && relax_ok_modulus(p->q);
             col.push_back(make_job(p, j.a, j.result, false));
         }
         for (size_t i : idx) col.push_back(make_job(jobs[i].plan, jobs[i].b, jobs[i].scratch, false));

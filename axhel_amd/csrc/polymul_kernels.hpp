@@ -11,7 +11,11 @@
 // E consecutive words and the inverse pass starts from exactly that layout,
 // so no LDS exchange sits between them.  Every step is the reference's
 // arithmetic on the same words, so the result is bit-identical to running
-// the five calls one after another.
+// the five calls one after another.  RLX = true runs the two forward
+// transforms on the relaxed schedule of fwd_bfly_relaxed (every modulus
+// below 2^61): their words differ on the way, but the normalised NTT(a) and
+// NTT(b) the product consumes are the unique residues, so the output is
+// unchanged.
 #pragma once
 
 #include "ntt_kernels.hpp"
@@ -51,7 +55,7 @@ struct FusedLayout {
     }
 };
 
-template <int LOG_G, int LOG_E, int GPW>
+template <int LOG_G, int LOG_E, int GPW, bool RLX = false>
 __global__ void __launch_bounds__(GPW << (LOG_G - LOG_E))
 polymul_row_kernel(PolyArgs a) {
     static_assert(FusedLayout<LOG_G, LOG_E>::ok(), "forward exit layout must be the inverse entry layout");
@@ -92,13 +96,13 @@ polymul_row_kernel(PolyArgs a) {
     });
     const TwGlobal<LOG_G, true, false> twf{P.twf, n, 0, o};
     const TwGlobal<LOG_G, false, false> twi{P.twi, n, 0, o};
-    run_chunks<LOG_G, LOG_E, GPW, true, false>(va, lds, g, tid, m, twf, false, fin);
+    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX>(va, lds, g, tid, m, twf, false, fin);
     static_for<0, E>([&](auto rc) {
         constexpr int r = decltype(rc)::value;
         vb[r] = ld_g(P.b1 + (row | (k0 + Gf::kr(0, r))));
     });
     __syncthreads();  // the LDS tile is reused by b's pass
-    run_chunks<LOG_G, LOG_E, GPW, true, false>(vb, lds, g, tid, m, twf, false, fin);
+    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX>(vb, lds, g, tid, m, twf, false, fin);
 
     // thread holds words row + kl + r, r in [0, E)
     const uint32_t kl = Gf::kt(Gf::NC - 1, tid);
@@ -151,7 +155,7 @@ polymul_row_kernel(PolyArgs a) {
 // a job.  Row o's forward and inverse twiddles are staged in LDS once for all
 // GPW rows (each row needs its own 2 x (2^LOG_G - 1) pairs, which the
 // per-limb kernel reads from L2).  Same arithmetic as polymul_row_kernel.
-template <int LOG_G, int LOG_E, int GPW>
+template <int LOG_G, int LOG_E, int GPW, bool RLX = false>
 __global__ void __launch_bounds__(GPW << (LOG_G - LOG_E))
 polymul_row_stack_kernel(PolyArgs a) {
     static_assert(FusedLayout<LOG_G, LOG_E>::ok(), "forward exit layout must be the inverse entry layout");
@@ -203,13 +207,13 @@ polymul_row_stack_kernel(PolyArgs a) {
     __syncthreads();  // publishes the twiddles
     const TwLds<LOG_G, true> twf{twf_lds};
     const TwLds<LOG_G, false> twi{twi_lds};
-    run_chunks<LOG_G, LOG_E, GPW, true, false>(va, lds, g, tid, m, twf, false, fin);
+    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX>(va, lds, g, tid, m, twf, false, fin);
     static_for<0, E>([&](auto rc) {
         constexpr int r = decltype(rc)::value;
         vb[r] = valid ? ld_g(b1 + row + k0 + Gf::kr(0, r)) : 0;
     });
     __syncthreads();  // the LDS tile is reused by b's pass
-    run_chunks<LOG_G, LOG_E, GPW, true, false>(vb, lds, g, tid, m, twf, false, fin);
+    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX>(vb, lds, g, tid, m, twf, false, fin);
 
     const uint32_t kl = Gf::kt(Gf::NC - 1, tid);
     uint64_t cv[E];
