@@ -2,9 +2,12 @@
 
 For each degree it times every compiled configuration (radix = words per
 thread, groups per workgroup, pass split) on a synthetic batch sized to fill
-the GPU, forward and inverse, and keeps the fastest.  The result is a small
-text table, one `log2(N) index` line per degree, that the library reads at
-start-up (`axhel_amd/ntt_tuning_gfx950.txt`, or the file named by
+the GPU, forward and inverse, and keeps the fastest for batches; it also
+times single-limb calls (the ComputeForward / ComputeInverse drop-ins, one
+synchronous call at a time), whose latency favours other shapes, and keeps
+the fastest of those for single calls.  The result is a small text table,
+one `log2(N) batch-index single-index` line per degree, that the library
+reads at start-up (`axhel_amd/ntt_tuning_gfx950.txt`, or the file named by
 AXHEL_HIP_TUNING).
 
 Usage: python -m axhel_amd.autotune [--logn 12 13 ...] [--out PATH] [--words W]
@@ -59,6 +62,33 @@ def tune_degree(logn, torch, words, reps, limbs_per_prime=64):
     return best["index"], results
 
 
+def tune_single(logn, torch, calls=200):
+    """Single-limb call latency per configuration: forward and inverse plain
+    calls on device buffers, each followed by a synchronisation."""
+    n = 1 << logn
+    q = params.find_ntt_prime(n, 50)
+    plan = ax.NTT(n, q, params.find_primitive_2n_root(n, q))
+    src = torch.randint(0, q, (n,), dtype=torch.int64, device="cuda")
+    dst = torch.empty_like(src)
+    results = []
+    for ci in range(ax.config_count(n)):
+        ax.config_select(n, ci)
+        us = 0.0
+        for call in (plan.ComputeForward, plan.ComputeInverse):
+            for _ in range(calls // 4):
+                call(dst, src, 1, 1)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                call(dst, src, 1, 1)
+                torch.cuda.synchronize()
+            us += (time.perf_counter() - t0) * 1e6 / calls
+        results.append({"index": ci, "config": ax.config_get(n, ci), "fwd_plus_inv_us": us})
+        print(json.dumps({"logn": logn, "single": True, **results[-1]}), flush=True)
+    ax.config_select(n, -1)
+    return min(results, key=lambda r: r["fwd_plus_inv_us"])["index"], results
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--logn", type=int, nargs="*", default=list(range(10, 18)))
@@ -72,15 +102,16 @@ def main():
     if os.path.exists(args.out):
         for line in open(args.out):
             parts = line.split()
-            if len(parts) == 2 and not line.startswith("#"):
-                table[int(parts[0])] = int(parts[1])
+            if len(parts) in (2, 3) and not line.startswith("#"):
+                table[int(parts[0])] = [int(x) for x in parts[1:]]
     for logn in args.logn:
         best, _ = tune_degree(logn, torch, args.words, args.reps)
-        table[logn] = best
+        single, _ = tune_single(logn, torch)
+        table[logn] = [best, single]
     with open(args.out, "w") as f:
-        f.write("# log2(N) config-index, chosen by axhel_amd.autotune on gfx950\n")
+        f.write("# log2(N) batch-config-index single-call-config-index, chosen by axhel_amd.autotune on gfx950\n")
         for logn in sorted(table):
-            f.write(f"{logn} {table[logn]}\n")
+            f.write(f"{logn} {' '.join(map(str, table[logn]))}\n")
     print("wrote", args.out, table, file=sys.stderr)
 
 

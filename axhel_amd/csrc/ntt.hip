@@ -218,8 +218,9 @@ std::vector<std::vector<NttConfig>> build_registry() {
     r[9] = {single<9, 4, 8>(), single<9, 3, 4>()};
     r[10] = {single<10, 4, 4>(), single<10, 3, 2>()};
     r[11] = {single<11, 4, 2>(), single<11, 3, 1>()};
-    r[12] = {single<12, 4, 1>(), single<12, 3, 1>()};
-    r[13] = {single<13, 4, 1>(), single<13, 3, 1>()};
+    // two-pass forms of 2^12 and 2^13: more, shorter workgroups, for single calls
+    r[12] = {single<12, 4, 1>(), single<12, 3, 1>(), two<6, 3, 8, 6, 3, 8>()};
+    r[13] = {single<13, 4, 1>(), single<13, 3, 1>(), two<6, 3, 16, 7, 3, 16>(), two<6, 3, 8, 7, 3, 8>()};
     r[14] = {two<7, 4, 32, 7, 4, 32>(), two<7, 4, 16, 7, 4, 16>(), two<7, 3, 16, 7, 3, 16>(),
              two<7, 3, 32, 7, 3, 16>()};
     r[15] = {two<8, 4, 16, 7, 4, 32>(), two<7, 4, 32, 8, 4, 16>(), two<8, 3, 16, 7, 3, 16>(),
@@ -235,9 +236,17 @@ std::vector<std::vector<NttConfig>> build_registry() {
 std::once_flag g_reg_once;
 std::vector<std::vector<NttConfig>> *g_reg = nullptr;
 std::atomic<int> g_selected[kMaxLogN + 1];
+// configuration of single-limb calls (the plain per-call path); -1: g_selected
+std::atomic<int> g_single[kMaxLogN + 1];
+// the tuning table's choices (written once, before any selection), which
+// config_select(-1) restores
+int g_tuned[kMaxLogN + 1];
+int g_tuned_single[kMaxLogN + 1];
 
 // Tuned configuration table written by axhel_amd/autotune.py: lines of
-// "log2(N) index".  Read from $AXHEL_HIP_TUNING, else from
+// "log2(N) index [single-call index]" (the index batches use, and optionally
+// the one single-limb calls use, whose latency favours other shapes).  Read
+// from $AXHEL_HIP_TUNING, else from
 // ntt_tuning_gfx950.txt in the package directory above this library.
 void load_tuning(const std::vector<std::vector<NttConfig>> &reg) {
     std::string path;
@@ -260,10 +269,12 @@ void load_tuning(const std::vector<std::vector<NttConfig>> &reg) {
     char line[256];
     while (std::fgets(line, sizeof(line), f)) {
         if (line[0] == '#') continue;
-        int logn = -1, idx = -1;
-        if (std::sscanf(line, "%d %d", &logn, &idx) == 2 && logn >= 1 && logn <= (int)kMaxLogN && idx >= 0 &&
-            idx < (int)reg[logn].size())
-            g_selected[logn].store(idx);
+        int logn = -1, idx = -1, single = -1;
+        const int got = std::sscanf(line, "%d %d %d", &logn, &idx, &single);
+        if (got >= 2 && logn >= 1 && logn <= (int)kMaxLogN && idx >= 0 && idx < (int)reg[logn].size()) {
+            g_tuned[logn] = idx;
+            if (got == 3 && single >= 0 && single < (int)reg[logn].size()) g_tuned_single[logn] = single;
+        }
     }
     std::fclose(f);
 }
@@ -271,8 +282,15 @@ void load_tuning(const std::vector<std::vector<NttConfig>> &reg) {
 const std::vector<std::vector<NttConfig>> &registry() {
     std::call_once(g_reg_once, [] {
         g_reg = new std::vector<std::vector<NttConfig>>(build_registry());
-        for (auto &s : g_selected) s.store(0);
+        for (uint32_t l = 0; l <= kMaxLogN; ++l) {
+            g_tuned[l] = 0;
+            g_tuned_single[l] = -1;
+        }
         load_tuning(*g_reg);
+        for (uint32_t l = 0; l <= kMaxLogN; ++l) {
+            g_selected[l].store(g_tuned[l]);
+            g_single[l].store(g_tuned_single[l]);
+        }
     });
     return *g_reg;
 }
@@ -313,6 +331,13 @@ const NttConfig &config_for(uint32_t logn) {
     int i = g_selected[logn].load();
     if (i < 0 || i >= (int)v.size()) i = 0;
     return v[i];
+}
+
+const NttConfig &config_for_single(uint32_t logn) {
+    const auto &v = configs_for(logn);
+    const int i = g_single[logn].load();
+    if (i >= 0 && i < (int)v.size()) return v[i];
+    return config_for(logn);
 }
 
 hipError_t prepare_config(const NttConfig &cfg) {
@@ -496,8 +521,8 @@ axhel_hip_status run_one(const axhel_hip_plan *plan, uint64_t *result, const uin
     AXHEL_HIP_TRY(stg.output(result, plan->n, &dres));
     const NttJobDev job = make_job(plan, dop, dres, inverse);
     const bool vec16 = aligned16(dop) && aligned16(dres);
-    AXHEL_HIP_TRY(launch_ntt(config_for(plan->logn), plan->logn, inverse, normalize, nullptr, job, 1, st, -1, vec16,
-                             nullptr, relax_ok_modulus(plan->q)));
+    AXHEL_HIP_TRY(launch_ntt(config_for_single(plan->logn), plan->logn, inverse, normalize, nullptr, job, 1, st, -1,
+                             vec16, nullptr, relax_ok_modulus(plan->q)));
     AXHEL_HIP_TRY(stg.finish());
     return AXHEL_HIP_OK;
 }
@@ -607,7 +632,13 @@ axhel_hip_status axhel_hip_ntt_config_select(uint64_t degree, int index) {
     if (!valid_degree(degree, &logn)) return AXHEL_HIP_INVALID_ARGUMENT;
     const auto &v = configs_for(logn);
     if (index < -1 || index >= (int)v.size()) return AXHEL_HIP_INVALID_ARGUMENT;
-    g_selected[logn].store(index < 0 ? 0 : index);  // -1: the first (built-in default) configuration
+    if (index < 0) {  // -1: back to the tuning table's choices (the first configuration without one)
+        g_selected[logn].store(g_tuned[logn]);
+        g_single[logn].store(g_tuned_single[logn]);
+    } else {
+        g_selected[logn].store(index);
+        g_single[logn].store(-1);  // single calls follow an explicit selection
+    }
     return AXHEL_HIP_OK;
 }
 
@@ -616,6 +647,14 @@ int axhel_hip_ntt_config_selected(uint64_t degree) {
     if (!valid_degree(degree, &logn)) return -1;
     (void)registry();
     return g_selected[logn].load();
+}
+
+int axhel_hip_ntt_config_single(uint64_t degree) {
+    uint32_t logn = 0;
+    if (!valid_degree(degree, &logn)) return -1;
+    (void)registry();
+    const int i = g_single[logn].load();
+    return i >= 0 ? i : g_selected[logn].load();
 }
 
 }  // extern "C"

@@ -74,8 +74,11 @@ bool build_stacks(const axhel_dev::NttJobDev *jobs, uint32_t count, int gpw, std
 
 // Compiled configurations for log2(degree) (empty for unsupported degrees).
 const std::vector<NttConfig> &configs_for(uint32_t logn);
-// Configuration currently selected for log2(degree).
+// Configuration currently selected for log2(degree) (batches, every multi-limb path).
 const NttConfig &config_for(uint32_t logn);
+// Configuration of single-limb calls: the tuning table's latency choice, or
+// config_for's once a configuration has been selected explicitly.
+const NttConfig &config_for_single(uint32_t logn);
 
 // Raises per-kernel limits (dynamic LDS above 64 KiB) for every pass of a
 // configuration; done before graph capture so capture records launches only.

@@ -350,10 +350,17 @@ typedef struct axhel_hip_ntt_config {
 /* Number of compiled configurations for a degree, and the i-th of them. */
 int axhel_hip_ntt_config_count(uint64_t degree);
 axhel_hip_status axhel_hip_ntt_config_get(uint64_t degree, int index, axhel_hip_ntt_config *config);
-/* Selects the configuration used for a degree from now on (process-wide);
-   index -1 restores the built-in default. */
+/* Selects the configuration used for a degree from now on (process-wide),
+   by batches and single calls alike; index -1 restores the tuning table's
+   choices (ntt_tuning_gfx950.txt, or $AXHEL_HIP_TUNING; configuration 0 for
+   a degree the table does not list). */
 axhel_hip_status axhel_hip_ntt_config_select(uint64_t degree, int index);
 int axhel_hip_ntt_config_selected(uint64_t degree);
+/* The configuration single-limb calls (axhel_hip_ntt_forward / _inverse, the
+   ComputeForward / ComputeInverse drop-ins) use: the tuning table's
+   single-call entry for the degree (a shape chosen for latency), or the
+   selected configuration once one has been selected. */
+int axhel_hip_ntt_config_single(uint64_t degree);
 
 #ifdef __cplusplus
 }

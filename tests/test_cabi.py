@@ -89,3 +89,28 @@ def test_rns_create_rejects_bad_arguments():
         axhel_amd.RNS(1 << 10, [97])  # 97 - 1 is not a multiple of 2048
     with pytest.raises(axhel_amd.AxhelError):
         axhel_amd.RNS(1 << 10, [])
+
+
+def test_tuning_table_sets_batch_and_single_call_configurations():
+    """The shipped table's two columns are what the library reports; an
+    explicit selection applies to single calls too, and -1 restores the
+    table's choices.  Host-side state only, no compute."""
+    path = os.path.join(ROOT, "axhel_amd", "ntt_tuning_gfx950.txt")
+    table = {}
+    for line in open(path):
+        parts = line.split()
+        if parts and not line.startswith("#"):
+            table[int(parts[0])] = [int(x) for x in parts[1:]]
+    assert all(len(v) == 2 for v in table.values())
+    for logn, (batch, single) in table.items():
+        n = 1 << logn
+        assert axhel_amd.config_selected(n) == batch and axhel_amd.config_single(n) == single, logn
+    n = 1 << 16
+    try:
+        axhel_amd.config_select(n, 2)
+        assert axhel_amd.config_selected(n) == 2 and axhel_amd.config_single(n) == 2
+    finally:
+        axhel_amd.config_select(n, -1)
+    assert axhel_amd.config_selected(n) == table[16][0] and axhel_amd.config_single(n) == table[16][1]
+    # a degree the table does not list: the first configuration for both
+    assert axhel_amd.config_selected(1 << 10) == 0 and axhel_amd.config_single(1 << 10) == 0
