@@ -40,8 +40,79 @@ int cu_count() {
     return cache[dev];
 }
 
+namespace {
+
+// Idle staging buffers by device, sizes rounded up to powers of two (64 KiB
+// at least).  A per-call hipMalloc / hipFree costs more than the transform of
+// a limb (hipFree waits for the device), so buffers are kept; at most
+// kPoolCap bytes stay idle.  The pool is never destroyed: no HIP call runs
+// during static destruction at exit.
+struct StagePool {
+    struct Buf {
+        int device;
+        size_t bytes;
+        void *p;
+    };
+    std::mutex mu;
+    std::vector<Buf> idle;
+    size_t idle_bytes = 0;
+};
+
+constexpr size_t kPoolCap = size_t(256) << 20;
+
+StagePool &stage_pool() {
+    static StagePool *pool = new StagePool;
+    return *pool;
+}
+
+size_t pooled_size(size_t bytes) {
+    size_t b = size_t(64) << 10;
+    while (b < bytes) b <<= 1;
+    return b;
+}
+
+hipError_t pool_acquire(int device, size_t bytes, void **p) {
+    StagePool &pool = stage_pool();
+    {
+        std::lock_guard<std::mutex> lk(pool.mu);
+        for (size_t i = 0; i < pool.idle.size(); ++i) {
+            if (pool.idle[i].device == device && pool.idle[i].bytes == bytes) {
+                *p = pool.idle[i].p;
+                pool.idle_bytes -= bytes;
+                pool.idle[i] = pool.idle.back();
+                pool.idle.pop_back();
+                return hipSuccess;
+            }
+        }
+    }
+    return hipMalloc(p, bytes);
+}
+
+void pool_release(int device, size_t bytes, void *p) {
+    StagePool &pool = stage_pool();
+    std::vector<void *> drop;
+    {
+        std::lock_guard<std::mutex> lk(pool.mu);
+        pool.idle.push_back({device, bytes, p});
+        pool.idle_bytes += bytes;
+        while (pool.idle_bytes > kPoolCap && !pool.idle.empty()) {  // oldest first
+            drop.push_back(pool.idle.front().p);
+            pool.idle_bytes -= pool.idle.front().bytes;
+            pool.idle.erase(pool.idle.begin());
+        }
+    }
+    for (void *d : drop) (void)hipFree(d);
+}
+
+}  // namespace
+
 Staging::~Staging() {
-    for (auto &e : entries_) (void)hipFree(e.dev);
+    if (entries_.empty()) return;
+    // An error path may leave copies or kernels queued on the buffers.  If
+    // even this wait fails the stream may still own them: they are dropped,
+    // not pooled (a device error ends the process's GPU work anyway).
+    if (!synced_ && hipStreamSynchronize(stream_) != hipSuccess) return;
+    for (auto &e : entries_) pool_release(e.device, e.bytes, e.dev);
 }
 
 hipError_t Staging::stage(const void *p, uint64_t n, bool copy_in, void **dev, Entry **entry) {
@@ -52,17 +123,19 @@ hipError_t Staging::stage(const void *p, uint64_t n, bool copy_in, void **dev, E
             return hipSuccess;
         }
     }
-    void *d = nullptr;
-    hipError_t err = hipMalloc(&d, n ? n * sizeof(uint64_t) : sizeof(uint64_t));
+    int device = 0;
+    hipError_t err = hipGetDevice(&device);
     if (err != hipSuccess) return err;
+    const size_t bytes = pooled_size(n * sizeof(uint64_t));
+    void *d = nullptr;
+    err = pool_acquire(device, bytes, &d);
+    if (err != hipSuccess) return err;
+    // registered before the copy, so every exit path returns it to the pool
+    entries_.push_back(Entry{p, d, n, false, bytes, device});
     if (copy_in && n) {
         err = hipMemcpyAsync(d, p, n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_);
-        if (err != hipSuccess) {
-            (void)hipFree(d);
-            return err;
-        }
+        if (err != hipSuccess) return err;
     }
-    entries_.push_back(Entry{p, d, n, false});
     *dev = d;
     *entry = &entries_.back();
     return hipSuccess;
@@ -104,7 +177,9 @@ hipError_t Staging::finish() {
                                         hipMemcpyDeviceToHost, stream_);
         if (err != hipSuccess) return err;
     }
-    return hipStreamSynchronize(stream_);
+    const hipError_t err = hipStreamSynchronize(stream_);
+    synced_ = err == hipSuccess;
+    return err;
 }
 
 }  // namespace axhel_host

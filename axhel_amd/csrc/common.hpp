@@ -42,7 +42,9 @@ inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 
 // Staging of host operands for the synchronous host-pointer path of the
 // entry points.  Device pointers pass through untouched; each distinct host
 // pointer gets one device buffer (exact aliasing between an operand and the
-// result therefore shares a buffer, as in-place calls need).
+// result therefore shares a buffer, as in-place calls need).  Buffers come
+// from a process-wide pool and go back to it once the call's stream has
+// synchronised, so a steady stream of host-pointer calls allocates nothing.
 class Staging {
 public:
     explicit Staging(hipStream_t s) : stream_(s) {}
@@ -63,10 +65,13 @@ private:
         void *dev;
         uint64_t n;
         bool is_out;
+        size_t bytes;  // pooled size
+        int device;
     };
     hipError_t stage(const void *p, uint64_t n, bool copy_in, void **dev, Entry **entry);
     std::vector<Entry> entries_;
     hipStream_t stream_;
+    bool synced_ = false;  // finish() synchronised the stream: buffers are idle
 };
 
 }  // namespace axhel_host

@@ -232,3 +232,49 @@ def test_relaxed_schedule_boundary_moduli(logn):
         for k in range(2):
             assert np.array_equal(host(res[k, l]), want[l][k]), ("rns", l, k)
     ctx.close()
+
+
+def test_host_pointer_calls_reuse_pooled_staging_buffers():
+    """Host-pointer calls stage through pooled device buffers, handed back
+    after each call's synchronisation: calls of changing sizes in a row, in
+    place and out of place, equal the oracle; then two threads issue
+    host-pointer calls at once (distinct buffers from the shared pool)."""
+    import threading
+    sizes = [1 << 10, 1 << 16, 1 << 12, 1 << 17, 1 << 16, 1 << 10, 1 << 17]
+    plans = {}
+    for i, n in enumerate(sizes):
+        if n not in plans:
+            q, psi = ntt_params(n, 50)
+            plans[n] = (ax.NTT(n, q, psi), O.Tables(n, q, psi), q)
+        ntt, t, q = plans[n]
+        a = O.MT19937_64(SEED + i).random_vector(n, 4 * q)
+        want = O.compute_forward(a, t, 1, 1)
+        out = np.zeros(n, dtype=np.uint64)
+        ntt.ComputeForward(out, a, 1, 1)
+        assert np.array_equal(out, want), (i, n)
+        h = a.copy()
+        ntt.ComputeForward(h, h, 1, 4)
+        assert np.array_equal(h, O.compute_forward(a, t, 1, 4)), (i, n)
+    n = 1 << 16
+    ntt, t, q = plans[n]
+    inputs = [O.MT19937_64(SEED + 100 + k).random_vector(n, 4 * q) for k in range(2)]
+    wants = [O.compute_forward(a, t, 1, 1) for a in inputs]
+    errors = []
+
+    def worker(k):
+        try:
+            for _ in range(20):
+                out = np.zeros(n, dtype=np.uint64)
+                ntt.ComputeForward(out, inputs[k], 1, 1)
+                if not np.array_equal(out, wants[k]):
+                    errors.append(k)
+                    return
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    ths = [threading.Thread(target=worker, args=(k,)) for k in range(2)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    assert not errors, errors
