@@ -36,7 +36,7 @@ def time_batch(batch, torch, reps):
 
 def tune_degree(logn, torch, words, reps, limbs_per_prime=64):
     n = 1 << logn
-    limbs = max(1, words // n)
+    limbs = max(1, min(words // n, 16384))  # small degrees: a bounded job list, not words / n jobs
     nprimes = max(1, min(24, limbs // limbs_per_prime)) if limbs >= limbs_per_prime else 1
     primes = params.rns_primes(n, nprimes)
     plans = [ax.NTT(n, q, params.find_primitive_2n_root(n, q)) for q in primes]

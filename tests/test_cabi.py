@@ -113,4 +113,5 @@ def test_tuning_table_sets_batch_and_single_call_configurations():
         axhel_amd.config_select(n, -1)
     assert axhel_amd.config_selected(n) == table[16][0] and axhel_amd.config_single(n) == table[16][1]
     # a degree the table does not list: the first configuration for both
-    assert axhel_amd.config_selected(1 << 10) == 0 and axhel_amd.config_single(1 << 10) == 0
+    assert 9 not in table
+    assert axhel_amd.config_selected(1 << 9) == 0 and axhel_amd.config_single(1 << 9) == 0
