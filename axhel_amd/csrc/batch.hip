@@ -71,7 +71,8 @@ hipError_t launch_group(const axhel_hip_batch *b, const axhel_hip_batch::Group &
                         int only_pass = -1) {
     static const NttJobDev none{};
     const StackTable st = stack_of(b, g);
-    return launch_ntt(config_for(g.logn), g.logn, g.inverse, g.normalize, b->d_jobs + g.offset, none, g.count,
+    return launch_ntt(config_for_launch(g.logn, g.count), g.logn, g.inverse, g.normalize, b->d_jobs + g.offset, none,
+                      g.count,
                       stream, only_pass, g.vec16, &st, g.relax_ok);
 }
 
@@ -181,7 +182,7 @@ axhel_hip_status axhel_hip_ntt_batch_create(const axhel_hip_ntt_job *jobs, size_
     // Stack tables (after all plain descriptors) for groups whose ROW pass
     // can run over same-plan stacks.
     for (auto &g : b->groups) {
-        const int gpw = stack_gpw(g.logn);
+        const int gpw = stack_gpw_of(config_for_launch(g.logn, g.count));
         std::vector<NttJobDev> st;
         uint32_t stacks = 0;
         if (gpw && build_stacks(host.data() + g.offset, g.count, gpw, st, stacks)) {
@@ -203,7 +204,7 @@ axhel_hip_status axhel_hip_ntt_batch_create(const axhel_hip_ntt_job *jobs, size_
             e = hipMemcpy(b->d_jobs, host.data(), host.size() * sizeof(NttJobDev), hipMemcpyHostToDevice);
     }
     for (const auto &g : b->groups)
-        if (e == hipSuccess) e = prepare_config(config_for(g.logn));
+        if (e == hipSuccess) e = prepare_config(config_for_launch(g.logn, g.count));
     if (e == hipSuccess && count) e = capture(b);
     if (e != hipSuccess) {
         set_last_error("batch create", e);
@@ -230,21 +231,22 @@ axhel_hip_status axhel_hip_ntt_batch_launch_eager(axhel_hip_batch *batch, void *
 int axhel_hip_ntt_batch_kernel_count(const axhel_hip_batch *batch) {
     if (!batch) return 0;
     int k = 0;
-    for (const auto &g : batch->groups) k += config_for(g.logn).npass;
+    for (const auto &g : batch->groups) k += config_for_launch(g.logn, g.count).npass;
     return k;
 }
 
 int axhel_hip_ntt_batch_stacked_groups(const axhel_hip_batch *batch) {
     if (!batch) return 0;
     int k = 0;
-    for (const auto &g : batch->groups) k += g.stack_gpw != 0 && g.stack_gpw == stack_gpw(g.logn);
+    for (const auto &g : batch->groups)
+        k += g.stack_gpw != 0 && g.stack_gpw == stack_gpw_of(config_for_launch(g.logn, g.count));
     return k;
 }
 
 axhel_hip_status axhel_hip_ntt_batch_launch_kernel(axhel_hip_batch *batch, int index, void *stream) {
     if (!batch || index < 0) return AXHEL_HIP_INVALID_ARGUMENT;
     for (const auto &g : batch->groups) {
-        const NttConfig &cfg = config_for(g.logn);
+        const NttConfig &cfg = config_for_launch(g.logn, g.count);
         if (index < cfg.npass) {
             AXHEL_HIP_TRY(launch_group(batch, g, as_stream(stream), index));
             return AXHEL_HIP_OK;

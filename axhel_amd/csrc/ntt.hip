@@ -340,6 +340,10 @@ const NttConfig &config_for_single(uint32_t logn) {
     return config_for(logn);
 }
 
+const NttConfig &config_for_launch(uint32_t logn, uint64_t count) {
+    return count <= (kLatencyWords >> logn) ? config_for_single(logn) : config_for(logn);
+}
+
 hipError_t prepare_config(const NttConfig &cfg) {
     for (int p = 0; p < cfg.npass; ++p) {
         hipError_t e = ensure_lds(cfg.fwd[p]);
@@ -358,8 +362,7 @@ hipError_t prepare_config(const NttConfig &cfg) {
 
 int stack_gpw(uint32_t logn) {
     if (configs_for(logn).empty()) return 0;
-    const NttConfig &c = config_for(logn);
-    return c.fwd_stack.fn ? c.fwd_stack.gpw : 0;
+    return stack_gpw_of(config_for(logn));
 }
 
 bool build_stacks(const NttJobDev *jobs, uint32_t count, int gpw, std::vector<NttJobDev> &out, uint32_t &stacks) {

@@ -201,13 +201,13 @@ axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const u
     // lazy only for output factor 4 (forward) / 2 (inverse), as ntt.cpp:173-204
     const bool normalize = inverse ? output_mod_factor != 2 : output_mod_factor != 4;
     const hipStream_t st = as_stream(stream);
-    const NttConfig &cfg = config_for(ctx->logn);
-    AXHEL_HIP_TRY(prepare_config(cfg));
     const uint32_t cnt = (uint32_t)count, L = ctx->limbs;
     const uint64_t nj = (uint64_t)cnt * L;
+    const NttConfig &cfg = config_for_launch(ctx->logn, nj);
+    AXHEL_HIP_TRY(prepare_config(cfg));
     // stacked ROW pass when padding each limb's polynomials to whole stacks
     // leaves at most one slot in eight empty (the batch rule)
-    const int gpw = stack_gpw(ctx->logn);
+    const int gpw = stack_gpw_of(cfg);
     uint32_t slots_per_limb = 0;
     if (gpw) {
         const uint64_t spl = (cnt + (uint64_t)gpw - 1) / gpw * gpw;

@@ -265,7 +265,7 @@ hipError_t launch_batch(axhel_hip_server *s, Slot &sl) {
     size_t pi = 0;
     for (auto &kv : ntt) {
         const uint32_t logn = std::get<0>(kv.first);
-        const NttConfig &cfg = config_for(logn);
+        const NttConfig &cfg = config_for_launch(logn, kv.second.size());
         if ((e = prepare_config(cfg)) != hipSuccess) return e;
         bool relax_ok = true;
         for (const NttJobDev &j : kv.second) relax_ok = relax_ok && relax_ok_modulus(j.q);
