@@ -95,3 +95,20 @@ def test_small_gpu_run_writes_the_reference_schema(tmp_path):
     assert list(raw[0].keys()) == sb.RAW_COLUMNS and len(raw) == 6 * len(rows)
     meta = open(os.path.join(out, "metadata.txt")).read()
     assert "counter_unit=nanoseconds" in meta
+
+
+@pytest.mark.gpu
+def test_host_timer_batch_subset_and_shuffle(tmp_path):
+    pytest.importorskip("torch")
+    out = str(tmp_path / "gpu2")
+    kernels = ["add_vs", "fma_f8", "reduce_barrett", "ntt_inverse_lazy"]
+    assert sb.main(["--degrees", "4096,2048", "--bits", "40,61", "--repetitions", "5", "--warmup", "1",
+                    "--batch", "3", "--timer", "host", "--kernels", ",".join(kernels), "--shuffle-scenarios",
+                    "--shuffle-seed", "7", "--no-raw", "--output-dir", out]) == 0
+    rows = list(csv.DictReader(open(os.path.join(out, "summary.csv"))))
+    assert len(rows) == 4 * len(kernels)
+    assert {(r["poly_degree"], r["qi_bits"]) for r in rows} == {(d, b) for d in ("4096", "2048") for b in ("40", "61")}
+    assert all(r["kernel"] in kernels and float(r["median_cycles"]) > 0 for r in rows)
+    assert not os.path.exists(os.path.join(out, "raw_samples.csv"))
+    meta = open(os.path.join(out, "metadata.txt")).read()
+    assert "timer=host" in meta and "batch=3" in meta
