@@ -207,6 +207,15 @@ def serve_leg(ax, seconds):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def p99_summary(serving):
+    """The serving loop's p99 at each fixed offered load (Poisson arrivals at
+    a stated fraction of the measured capacity), lifted beside the headline."""
+    if not serving or "loads" not in serving:
+        return None
+    return [{"offered_req_per_s": x["offered_req_per_s"], "fraction_of_capacity": x["fraction"],
+             "p50_us": x["p50_us"], "p99_us": x["p99_us"], "mix": serving.get("mix")} for x in serving["loads"]]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -425,7 +434,7 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     out = {
-        "metric": "NTT limbs/sec at N=2^16,L=24 (batched vs plain)",
+        "metric": "NTT limbs/sec at N=2^16,L=24 (batched vs plain); p99 latency at fixed load",
         "value": value,
         "unit": "limbs/s",
         "n_gpus": 1,
@@ -459,6 +468,7 @@ def main():
         "polymul": pmul,
         "uneven_batch": uneven,
         "serving": serving,
+        "p99_latency": p99_summary(serving),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
