@@ -202,3 +202,61 @@ def test_batch_rejects_host_pointers():
         ax.Batch([(plan, d, h.ctypes.data, False, 1)])
     with pytest.raises(ax.AxhelError):
         ax.Batch([(plan, h.ctypes.data, d, False, 1)])
+
+
+def test_random_mixed_batches_equal_plain_calls():
+    """Seeded random batches: degrees 2^1..2^17, moduli of 20..62 bits (both
+    sides of the 2^61 schedule gate), both directions and output ranges,
+    in-place and out-of-place jobs, buffers only 8-byte aligned (the batch
+    then uses 8-byte accesses).  Every job equals the plain call; a sample
+    equals the oracle."""
+    torch = torch_mod()
+    rng = np.random.default_rng(2026)
+    for trial in range(3):
+        specs, total = [], 0
+        for _ in range(60):
+            logn = int(rng.integers(1, 18))
+            n = 1 << logn
+            bits = int(rng.integers(max(20, logn + 3), 63))
+            inverse = bool(rng.integers(0, 2))
+            lazy = bool(rng.integers(0, 2))
+            inplace = bool(rng.integers(0, 2))
+            specs.append((n, bits, inverse, lazy, inplace))
+            total += 2 * n + 1
+        plans = {}
+        for n, bits, *_ in specs:
+            if (n, bits) not in plans:
+                q = params.find_ntt_prime(n, bits)
+                psi = params.find_primitive_2n_root(n, q)
+                plans[(n, bits)] = (ax.NTT(n, q, psi), q, psi)
+        base = torch.zeros(total + 1, dtype=torch.int64, device="cuda")
+        arena = base[1:]  # one word in: every buffer is 8-byte, not 16-byte, aligned
+        ref_arena = torch.zeros_like(arena)
+        jobs, placed, off, inputs = [], [], 0, []
+        for n, bits, inverse, lazy, inplace in specs:
+            plan, q, psi = plans[(n, bits)]
+            f_in = 2 if inverse else 4
+            a = params.uniform(rng, n, f_in * q)
+            src = arena[off:off + n]
+            src.copy_(torch.from_numpy(a.view(np.int64)))
+            dst = src if inplace else arena[off + n:off + 2 * n]
+            rsrc = ref_arena[off:off + n]
+            rsrc.copy_(src)
+            rdst = rsrc if inplace else ref_arena[off + n:off + 2 * n]
+            of = (f_in if lazy else 1)
+            jobs.append((plan, dst, src, inverse, of))
+            placed.append((plan, q, psi, n, inverse, of, dst, rdst, rsrc, a))
+            off += 2 * n + 1
+        b = ax.Batch(jobs)
+        b.launch()
+        for plan, q, psi, n, inverse, of, dst, rdst, rsrc, a in placed:
+            (plan.ComputeInverse if inverse else plan.ComputeForward)(rdst, rsrc, 1, of)
+        torch.cuda.synchronize()
+        for i, (plan, q, psi, n, inverse, of, dst, rdst, rsrc, a) in enumerate(placed):
+            assert torch.equal(dst, rdst), (trial, i, n, q.bit_length(), inverse, of)
+        for i in range(0, len(placed), 7):
+            plan, q, psi, n, inverse, of, dst, rdst, rsrc, a = placed[i]
+            t = O.Tables(n, q, psi)
+            want = O.compute_inverse(a, t, 1, of) if inverse else O.compute_forward(a, t, 1, of)
+            assert np.array_equal(host(dst), want), (trial, i, n, q.bit_length(), inverse, of)
+        b.close()
