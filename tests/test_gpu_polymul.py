@@ -140,3 +140,52 @@ def test_polymul_stacked_every_config(logn):
             assert np.array_equal(host(da).reshape(J, n), A)
     finally:
         ax.config_select(n, -1)
+
+
+def test_random_mixed_polymul_jobs():
+    """Seeded random product jobs in one call: degrees 2^10..2^17 (single- and
+    two-pass fused kernels), moduli of 30..62 bits (both sides of the relaxed
+    schedule's 2^61 gate), with and without an addend, every buffer only
+    8-byte aligned.  Fused equals the five-call sequence on every job; a
+    sample equals the oracle."""
+    torch = torch_mod()
+    rng = np.random.default_rng(77)
+    specs, total = [], 0
+    for _ in range(40):
+        logn = int(rng.integers(10, 18))
+        specs.append((1 << logn, int(rng.integers(30, 63)), bool(rng.integers(0, 2))))
+        total += 6 * (1 << logn) + 1
+    plans = {}
+    for n, bits, _ in specs:
+        if (n, bits) not in plans:
+            q = params.find_ntt_prime(n, bits)
+            psi = params.find_primitive_2n_root(n, q)
+            plans[(n, bits)] = (ax.NTT(n, q, psi), q, psi)
+    base = torch.zeros(total + 1, dtype=torch.int64, device="cuda")
+    arena = base[1:]
+    outs = {}
+    placed = []
+    off = 0
+    for n, bits, has_c in specs:
+        plan, q, psi = plans[(n, bits)]
+        a, b, c = (params.uniform(rng, n, q) for _ in range(3))
+        views = [arena[off + k * n:off + (k + 1) * n] for k in range(6)]  # a, b, c, fused out, seq out, scratch
+        for v, x in zip(views[:3], (a, b, c)):
+            v.copy_(torch.from_numpy(x.view(np.int64)))
+        s = int(params.uniform(rng, 1, q)[0])
+        placed.append((plan, q, psi, n, has_c, s, a, b, c, views))
+        off += 6 * n + 1
+    for fused, slot in ((True, 3), (False, 4)):
+        pm = ax.Polymul([(p[0], p[9][slot], p[9][0], p[9][1], p[9][2] if p[4] else None, p[5], p[9][5])
+                         for p in placed], fused=fused)
+        pm.launch()
+        torch.cuda.synchronize()
+        pm.close()
+    for i, (plan, q, psi, n, has_c, s, a, b, c, views) in enumerate(placed):
+        assert torch.equal(views[3], views[4]), (i, n, q.bit_length(), has_c)
+    for i in range(0, len(placed), 6):
+        plan, q, psi, n, has_c, s, a, b, c, views = placed[i]
+        t = O.Tables(n, q, psi)
+        A, B = O.compute_forward(a, t, 1, 1), O.compute_forward(b, t, 1, 1)
+        D = O.fma_mod(O.mul_mod(A, B, q, 1), s, c if has_c else None, q, 1)
+        assert np.array_equal(host(views[3]), O.compute_inverse(D, t, 1, 1)), (i, n, q.bit_length(), has_c)
