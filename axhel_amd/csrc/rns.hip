@@ -4,14 +4,20 @@
 // limb, integration/seal/SEAL-4.4.5-AXHEL.patch:185-226); here one call
 // covers count x L limbs.
 //
-// Each call builds its job table on the device (a fill kernel into
-// stream-ordered scratch from hipMallocAsync, released with hipFreeAsync on
-// the same stream), so calls on different streams never share scratch and
-// no table crosses PCIe.  The transforms then run through the batched
-// kernels, the ROW pass stacked over polynomials when the stacks fill.
+// Each call builds its job table on the device (a fill kernel), so no table
+// crosses PCIe, into a table buffer the context keeps per stream: calls on
+// one stream run in order, so the buffer is free again for the next call on
+// that stream, and calls on different streams never share one.  A call that
+// repeats the previous NTT call on its stream (same buffers, count and
+// direction) finds its table in place and skips the fill.  The context's
+// mutex is held while a call enqueues its work, so two threads issuing on
+// one stream cannot interleave a fill with another call's passes.  The
+// transforms then run through the batched kernels, the ROW pass stacked over
+// polynomials when the stacks fill.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -41,6 +47,19 @@ struct axhel_hip_rns {
     std::vector<axhel_hip_plan *> plans;
     RnsLimbDev *d_info = nullptr;
     bool relax_ok = true;  // every modulus below 2^61
+    // job-table buffer per stream, and the NTT call whose table it holds
+    struct StreamTable {
+        hipStream_t stream;
+        void *buf;
+        size_t bytes;
+        bool ntt_valid;
+        const void *result, *operand;
+        uint64_t count;
+        int inverse;
+        uint32_t slots;  // stack slots per limb (0: no stack table)
+    };
+    std::mutex mu;
+    std::vector<StreamTable> tables;
 };
 
 namespace {
@@ -133,6 +152,35 @@ __global__ void rns_fill_elt(EltLimbDev *out, const RnsLimbDev *info, uint64_t *
 
 bool device_ptr(const void *p) { return p && is_device_pointer(p); }
 
+// The context's table buffer for stream st, at least `bytes` long (caller
+// holds ctx->mu).  Growing waits for the stream's queued work, which may
+// still read the old buffer, before freeing it; buffers only grow.
+hipError_t stream_table(axhel_hip_rns *ctx, hipStream_t st, size_t bytes, axhel_hip_rns::StreamTable **out) {
+    axhel_hip_rns::StreamTable *t = nullptr;
+    for (auto &e : ctx->tables)
+        if (e.stream == st) t = &e;
+    if (!t) {
+        ctx->tables.push_back({st, nullptr, 0, false, nullptr, nullptr, 0, 0, 0});
+        t = &ctx->tables.back();
+    }
+    if (t->bytes < bytes) {
+        if (t->buf) {
+            hipError_t e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return e;
+            (void)hipFree(t->buf);
+            t->buf = nullptr;
+            t->bytes = 0;
+        }
+        t->ntt_valid = false;
+        const size_t want = std::max(bytes, size_t(64) << 10);
+        hipError_t e = hipMalloc(&t->buf, want);
+        if (e != hipSuccess) return e;
+        t->bytes = want;
+    }
+    *out = t;
+    return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -182,6 +230,7 @@ axhel_hip_status axhel_hip_rns_create(uint64_t degree, const uint64_t *moduli, c
 
 axhel_hip_status axhel_hip_rns_destroy(axhel_hip_rns *ctx) {
     if (!ctx) return AXHEL_HIP_OK;
+    for (auto &t : ctx->tables) (void)hipFree(t.buf);
     for (auto *p : ctx->plans) axhel_hip_ntt_plan_destroy(p);
     (void)hipFree(ctx->d_info);
     delete ctx;
@@ -214,14 +263,32 @@ axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const u
         if ((spl * L - nj) * 8 <= spl * L) slots_per_limb = (uint32_t)spl;
     }
     const uint64_t entries = nj + (uint64_t)slots_per_limb * L;
-    void *scratch = nullptr;
-    AXHEL_HIP_TRY(hipMallocAsync(&scratch, entries * sizeof(NttJobDev), st));
-    NttJobDev *jobs = (NttJobDev *)scratch;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    axhel_hip_rns::StreamTable *tab_buf = nullptr;
+    AXHEL_HIP_TRY(stream_table(ctx, st, entries * sizeof(NttJobDev), &tab_buf));
+    NttJobDev *jobs = (NttJobDev *)tab_buf->buf;
     NttJobDev *stacks = slots_per_limb ? jobs + nj : nullptr;
-    const uint64_t fill = std::max<uint64_t>(nj, (uint64_t)slots_per_limb * L);
-    hipLaunchKernelGGL(rns_fill_ntt, dim3((unsigned)((fill + 255) / 256)), dim3(256), 0, st, jobs, stacks,
-                       ctx->d_info, result, operand, cnt, L, ctx->n, inverse, slots_per_limb);
-    hipError_t e = hipGetLastError();
+    hipError_t e = hipSuccess;
+    // the table depends on the buffers, the count, the direction and the
+    // stack layout (slots per limb, from the configuration's group count): a
+    // repeated call finds it in place
+    const bool same = tab_buf->ntt_valid && tab_buf->result == result && tab_buf->operand == operand &&
+                      tab_buf->count == count && tab_buf->inverse == inverse && tab_buf->slots == slots_per_limb;
+    if (!same) {
+        tab_buf->ntt_valid = false;
+        const uint64_t fill = std::max<uint64_t>(nj, (uint64_t)slots_per_limb * L);
+        hipLaunchKernelGGL(rns_fill_ntt, dim3((unsigned)((fill + 255) / 256)), dim3(256), 0, st, jobs, stacks,
+                           ctx->d_info, result, operand, cnt, L, ctx->n, inverse, slots_per_limb);
+        e = hipGetLastError();
+        if (e == hipSuccess) {
+            tab_buf->ntt_valid = true;
+            tab_buf->result = result;
+            tab_buf->operand = operand;
+            tab_buf->count = count;
+            tab_buf->inverse = inverse;
+            tab_buf->slots = slots_per_limb;
+        }
+    }
     if (e == hipSuccess) {
         static const NttJobDev none{};
         StackTable tab;
@@ -234,9 +301,7 @@ axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const u
         e = launch_ntt(cfg, ctx->logn, inverse != 0, normalize, jobs, none, (uint32_t)nj, st, -1, vec16,
                        stacks ? &tab : nullptr, ctx->relax_ok);
     }
-    const hipError_t fe = hipFreeAsync(scratch, st);
     AXHEL_HIP_TRY(e);
-    AXHEL_HIP_TRY(fe);
     return AXHEL_HIP_OK;
 }
 
@@ -310,16 +375,16 @@ axhel_hip_status axhel_hip_rns_eltwise(axhel_hip_rns *ctx, int op, uint64_t *res
             return AXHEL_HIP_INVALID_ARGUMENT;
     }
     const uint64_t nl = count * L;
-    void *scratch = nullptr;
-    AXHEL_HIP_TRY(hipMallocAsync(&scratch, nl * sizeof(EltLimbDev), st));
-    EltLimbDev *d = (EltLimbDev *)scratch;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    axhel_hip_rns::StreamTable *tab_buf = nullptr;
+    AXHEL_HIP_TRY(stream_table(ctx, st, nl * sizeof(EltLimbDev), &tab_buf));
+    tab_buf->ntt_valid = false;  // the element-wise table overwrites the buffer
+    EltLimbDev *d = (EltLimbDev *)tab_buf->buf;
     hipLaunchKernelGGL(rns_fill_elt, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st, d, ctx->d_info, result,
                        operand1, b, (uint32_t)count, L, ctx->n, sc);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = launch_elt_batch(eop, f, d, (uint32_t)nl, ctx->n, st);
-    const hipError_t fe = hipFreeAsync(scratch, st);
     AXHEL_HIP_TRY(e);
-    AXHEL_HIP_TRY(fe);
     return AXHEL_HIP_OK;
 }
 

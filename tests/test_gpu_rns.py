@@ -143,3 +143,61 @@ def test_rns_calls_on_two_streams():
     assert torch.equal(o2, torch.flip(r1, dims=[0]))
     ctx.close()
 
+
+
+def test_rns_cached_tables_follow_every_change():
+    """The context keeps one job table per stream and skips the fill when a
+    call repeats the previous NTT call on that stream.  On one stream, in
+    sequence: repeated identical calls, an element-wise call in between
+    (overwrites the table), the other direction, a smaller and a larger
+    count (the buffer grows), and a configuration change that alters the
+    stack layout; every result equals the batch computed independently."""
+    n, L = 1 << 16, 6
+    torch, qs, ctx, data, src = setup(n, L, 24, 31, 4)
+    plans = [ax.NTT(n, q, params.find_primitive_2n_root(n, q)) for q in qs]
+
+    def want(count, inverse, inp):
+        out = torch.full_like(inp, -1)
+        b = ax.Batch([(plans[l], out[p, l], inp[p, l], inverse, 1) for p in range(count) for l in range(L)])
+        b.launch_eager()
+        torch.cuda.synchronize()
+        b.close()
+        return out[:count]
+
+    fwd24 = want(24, False, src)
+    out = torch.full_like(src, -1)
+    for _ in range(3):  # identical calls: table reused
+        out.fill_(-1)
+        ctx.ntt(out, src, 24, False, 1)
+        torch.cuda.synchronize()
+        assert torch.equal(out, fwd24)
+    lo = torch.remainder(src, torch.tensor(qs, device="cuda").view(1, L, 1))
+    tmp = torch.empty_like(src)
+    ctx.eltwise(ax.OP_ADD, tmp, lo, lo, None, 24, 1, 1)  # overwrites the table buffer
+    out.fill_(-1)
+    ctx.ntt(out, src, 24, False, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(out, fwd24)
+    back = torch.full_like(src, -1)
+    ctx.ntt(back, fwd24.contiguous(), 24, True, 1)  # other direction, other buffers
+    torch.cuda.synchronize()
+    assert torch.equal(back, torch.remainder(src, torch.tensor(qs, device="cuda").view(1, L, 1)))
+    out.fill_(-1)
+    ctx.ntt(out, src, 5, False, 1)  # smaller count
+    torch.cuda.synchronize()
+    assert torch.equal(out[:5], fwd24[:5]) and bool((out[5:] == -1).all())
+    big = torch.cat([src, src], 0).contiguous()  # larger count: the buffer grows
+    big_out = torch.full_like(big, -1)
+    ctx.ntt(big_out, big, 48, False, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(big_out[:24], fwd24) and torch.equal(big_out[24:], fwd24)
+    try:
+        for ci in range(ax.config_count(n)):  # stack layouts of every configuration
+            ax.config_select(n, ci)
+            out.fill_(-1)
+            ctx.ntt(out, src, 24, False, 1)
+            torch.cuda.synchronize()
+            assert torch.equal(out, fwd24), ci
+    finally:
+        ax.config_select(n, -1)
+    ctx.close()
