@@ -614,10 +614,13 @@ axhel_hip_status axhel_hip_server_submit(axhel_hip_server *s, const axhel_hip_re
 axhel_hip_status axhel_hip_server_wait(axhel_hip_server *s, uint64_t ticket, int64_t timeout_us) {
     if (!s) return AXHEL_HIP_INVALID_ARGUMENT;
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end()) return AXHEL_HIP_INVALID_ARGUMENT;
-    Req *R = it->second;
-    auto pred = [&] { return R->state == ST_DONE; };
+    if (s->reqs.find(ticket) == s->reqs.end()) return AXHEL_HIP_INVALID_ARGUMENT;
+    // Looked up by ticket on every wake-up: another thread waiting on the same
+    // ticket may have collected (and freed) the request meanwhile.
+    auto pred = [&] {
+        auto f = s->reqs.find(ticket);
+        return f == s->reqs.end() || f->second->state == ST_DONE;
+    };
     ++s->active_waiters;
     bool done = true;
     if (timeout_us < 0) s->cv_done.wait(lk, pred);
@@ -625,8 +628,11 @@ axhel_hip_status axhel_hip_server_wait(axhel_hip_server *s, uint64_t ticket, int
     --s->active_waiters;
     s->cv_done.notify_all();
     if (!done) return AXHEL_HIP_TIMEOUT;
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return AXHEL_HIP_INVALID_ARGUMENT;  // collected by another waiter
+    Req *R = it->second;
     const axhel_hip_status st = R->status;
-    s->reqs.erase(R->id);
+    s->reqs.erase(it);
     delete R;
     return st;
 }

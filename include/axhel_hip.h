@@ -276,7 +276,9 @@ axhel_hip_status axhel_hip_server_submit(axhel_hip_server *server, const axhel_h
                                          uint64_t *ticket);
 /* Waits up to timeout_us (< 0: forever) for the request; on completion
    returns its final status and forgets the ticket.  AXHEL_HIP_TIMEOUT here
-   means "still running" (the ticket stays valid). */
+   means "still running" (the ticket stays valid).  When several threads wait
+   on one ticket, one of them collects the status and the others return
+   AXHEL_HIP_INVALID_ARGUMENT. */
 axhel_hip_status axhel_hip_server_wait(axhel_hip_server *server, uint64_t ticket, int64_t timeout_us);
 /* Cancels a queued request (AXHEL_HIP_OK), or reports AXHEL_HIP_REJECTED if
    it already runs or finished. */

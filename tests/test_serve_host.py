@@ -103,6 +103,23 @@ def test_wait_timeout_keeps_ticket():
     s.close()
 
 
+def test_two_waiters_on_one_ticket():
+    """Exactly one of two threads waiting on the same ticket collects the
+    status; the other learns that the ticket was collected."""
+    s = ax.Server(backend=ax.BACKEND_NULL, null_latency_us=60000, slots=1, memory_budget_bytes=64 * MIB)
+    for _ in range(5):
+        st, t, k = req(s)
+        assert st == ax.OK
+        got = []
+        ths = [threading.Thread(target=lambda: got.append(s.wait(t))) for _ in range(2)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        assert sorted(got) == sorted([ax.OK, ax.INVALID_ARGUMENT]), got
+    s.close()
+
+
 def test_staging_allocation_failures_are_reported():
     s = ax.Server(backend=ax.BACKEND_NULL, null_latency_us=100, memory_budget_bytes=256 * MIB, fail_every_alloc=4)
     keep, tickets = [], []
