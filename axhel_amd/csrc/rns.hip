@@ -181,6 +181,24 @@ hipError_t stream_table(axhel_hip_rns *ctx, hipStream_t st, size_t bytes, axhel_
     return hipSuccess;
 }
 
+// Calls on a capturing stream are refused: a captured call would depend on
+// the context's per-stream table buffer, which later calls overwrite (and a
+// call that skips the fill would record no table at all).  Batches are the
+// capturable form of the same work.
+axhel_hip_status refuse_capture(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const hipError_t e = hipStreamIsCapturing(st, &cs);
+    if (e != hipSuccess) {
+        set_last_error("rns: stream state", e);
+        return AXHEL_HIP_HIP_ERROR;
+    }
+    if (cs != hipStreamCaptureStatusNone) {
+        set_last_error_text("rns: calls cannot be captured into a graph; capture a batch (axhel_hip_ntt_batch_*)");
+        return AXHEL_HIP_INVALID_ARGUMENT;
+    }
+    return AXHEL_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -250,6 +268,7 @@ axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const u
     // lazy only for output factor 4 (forward) / 2 (inverse), as ntt.cpp:173-204
     const bool normalize = inverse ? output_mod_factor != 2 : output_mod_factor != 4;
     const hipStream_t st = as_stream(stream);
+    if (const axhel_hip_status cs = refuse_capture(st)) return cs;
     const uint32_t cnt = (uint32_t)count, L = ctx->limbs;
     const uint64_t nj = (uint64_t)cnt * L;
     const NttConfig &cfg = config_for_launch(ctx->logn, nj);
@@ -315,6 +334,7 @@ axhel_hip_status axhel_hip_rns_eltwise(axhel_hip_rns *ctx, int op, uint64_t *res
         return AXHEL_HIP_INVALID_ARGUMENT;
     const uint32_t L = ctx->limbs;
     const hipStream_t st = as_stream(stream);
+    if (const axhel_hip_status cs = refuse_capture(st)) return cs;
     EltFillArgs sc{};
     int eop = 0, f = 1;
     const uint64_t *b = nullptr;

@@ -296,8 +296,10 @@ axhel_hip_status axhel_hip_server_reset_stats(axhel_hip_server *server);
    reference's caller does with one ComputeForward/ComputeInverse or Eltwise*Mod
    call per limb (integration/seal/SEAL-4.4.5-AXHEL.patch:185-226: the limb
    loop); the arithmetic per limb is that call's.  Calls are asynchronous on
-   `stream`; their scratch is stream-ordered, so calls may run on several
-   streams at once.  No reference counterpart. */
+   `stream`; the context keeps one job-table buffer per stream, so calls may
+   run on several streams at once.  Calls on a stream that is being captured
+   into a graph are refused (AXHEL_HIP_INVALID_ARGUMENT): capture a batch
+   instead.  No reference counterpart. */
 typedef struct axhel_hip_rns axhel_hip_rns;
 
 #define AXHEL_HIP_RNS_MAX_LIMBS 64

@@ -201,3 +201,39 @@ def test_rns_cached_tables_follow_every_change():
     finally:
         ax.config_select(n, -1)
     ctx.close()
+
+
+def test_rns_calls_refuse_graph_capture():
+    """A captured call would depend on the context's per-stream table buffer,
+    which later calls overwrite: calls on a capturing stream are refused
+    before anything is enqueued, and the context works normally after."""
+    n, L, count = 1 << 12, 3, 2
+    torch, qs, ctx, data, src = setup(n, L, count, 41, 4)
+    out = torch.full_like(src, -1)
+    x = torch.ones(8, device="cuda")
+    y = torch.empty_like(x)
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    errors = []
+    with torch.cuda.graph(g, capture_error_mode="relaxed"):
+        y.copy_(x * 2)
+        for call in (lambda: ctx.ntt(out, src, count, False, 1, stream=torch.cuda.current_stream()),
+                     lambda: ctx.eltwise(ax.OP_ADD, out, src, src, None, count, 1, 1,
+                                         stream=torch.cuda.current_stream())):
+            try:
+                call()
+            except ax.AxhelError as e:
+                errors.append(str(e))
+    assert len(errors) == 2, errors
+    g.replay()
+    torch.cuda.synchronize()
+    assert bool((out == -1).all()) and bool((y == 2).all())  # nothing of ours was captured
+    ctx.ntt(out, src, count, False, 1)
+    ref = torch.full_like(src, -1)
+    plans = [ax.NTT(n, q, params.find_primitive_2n_root(n, q)) for q in qs]
+    for p in range(count):
+        for l in range(L):
+            plans[l].ComputeForward(ref[p, l], src[p, l], 1, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    ctx.close()
