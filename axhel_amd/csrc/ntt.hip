@@ -14,6 +14,7 @@
 
 #include "eltwise_host.hpp"
 #include "ntt_host.hpp"
+#include "ntt_registry.hpp"
 #include "polymul_kernels.hpp"
 
 using axhel_dev::NttJobDev;
@@ -103,133 +104,13 @@ uint64_t brv_host(uint64_t i, uint32_t logn) {
     return r;
 }
 
-// ------------------------------------------------------------ configurations
-
-template <int LG, int LE, int GPW, bool COL, bool MULTI, bool FWD>
-PassKernel pk() {
-    PassKernel k;
-    k.fn = reinterpret_cast<const void *>(&axhel_dev::ntt_pass_kernel<LG, LE, GPW, FWD, COL, MULTI>);
-    if constexpr (FWD)
-        k.fn_relaxed = reinterpret_cast<const void *>(&axhel_dev::ntt_pass_kernel<LG, LE, GPW, true, COL, MULTI, true>);
-    k.log_g = LG;
-    k.log_e = LE;
-    k.gpw = GPW;
-    k.col = COL;
-    k.multi = MULTI;
-    k.threads = GPW << (LG - LE);
-    k.lds_bytes = (size_t)axhel_dev::Lds<LG, LE, GPW, COL>::TOTAL_WORDS * sizeof(uint64_t);
-    return k;
-}
-
-template <int LG, int LE, int GPW>
-PassKernel pk_fused() {
-    PassKernel k;
-    k.fn = reinterpret_cast<const void *>(&axhel_dev::polymul_row_kernel<LG, LE, GPW>);
-    k.fn_relaxed = reinterpret_cast<const void *>(&axhel_dev::polymul_row_kernel<LG, LE, GPW, true>);
-    k.log_g = LG;
-    k.log_e = LE;
-    k.gpw = GPW;
-    k.col = false;
-    k.multi = false;
-    k.threads = GPW << (LG - LE);
-    k.lds_bytes = (size_t)axhel_dev::Lds<LG, LE, GPW, false>::WORDS * sizeof(uint64_t);
-    return k;
-}
-
-template <int LG, int LE, int GPW>
-PassKernel pk_fused_stack() {
-    PassKernel k;
-    k.fn = reinterpret_cast<const void *>(&axhel_dev::polymul_row_stack_kernel<LG, LE, GPW>);
-    k.fn_relaxed = reinterpret_cast<const void *>(&axhel_dev::polymul_row_stack_kernel<LG, LE, GPW, true>);
-    k.log_g = LG;
-    k.log_e = LE;
-    k.gpw = GPW;
-    k.col = false;
-    k.multi = false;
-    k.stack = true;
-    k.threads = GPW << (LG - LE);
-    k.lds_bytes = ((size_t)axhel_dev::Lds<LG, LE, GPW, false>::WORDS + 4 * ((1u << LG) - 1)) * sizeof(uint64_t);
-    return k;
-}
-
-template <int LG, int LE, int GPW, bool FWD>
-PassKernel pk_stack() {
-    PassKernel k;
-    k.fn = reinterpret_cast<const void *>(&axhel_dev::ntt_row_stack_kernel<LG, LE, GPW, FWD>);
-    if constexpr (FWD)
-        k.fn_relaxed = reinterpret_cast<const void *>(&axhel_dev::ntt_row_stack_kernel<LG, LE, GPW, true, true>);
-    k.log_g = LG;
-    k.log_e = LE;
-    k.gpw = GPW;
-    k.col = false;
-    k.multi = false;
-    k.stack = true;
-    k.threads = GPW << (LG - LE);
-    k.lds_bytes = ((size_t)axhel_dev::Lds<LG, LE, GPW, false>::WORDS + 2 * ((1u << LG) - 1)) * sizeof(uint64_t);
-    return k;
-}
-
-template <int LG, int LE, int GPW>
-NttConfig single() {
-    NttConfig c;
-    c.pub.log_elems = LE;
-    c.pub.row_groups = GPW;
-    c.pub.col_groups = 0;
-    c.pub.log_col = 0;
-    c.pub.log_elems_col = 0;
-    c.npass = 1;
-    c.fwd[0] = pk<LG, LE, GPW, false, (GPW > 1), true>();
-    c.inv[0] = pk<LG, LE, GPW, false, (GPW > 1), false>();
-    if constexpr (GPW == 1 && LG >= 12) c.fused = pk_fused<LG, LE, GPW>();
-    return c;
-}
-
-// Two passes: COL over the top LC bits, ROW over the low LR bits.
-template <int LC, int LEC, int GC, int LR, int LER, int GR>
-NttConfig two() {
-    NttConfig c;
-    c.pub.log_elems = LER;
-    c.pub.row_groups = GR;
-    c.pub.col_groups = GC;
-    c.pub.log_col = LC;
-    c.pub.log_elems_col = LEC;
-    c.npass = 2;
-    c.fwd[0] = pk<LC, LEC, GC, true, false, true>();
-    c.fwd[1] = pk<LR, LER, GR, false, false, true>();
-    c.inv[0] = pk<LR, LER, GR, false, false, false>();
-    c.inv[1] = pk<LC, LEC, GC, true, false, false>();
-    c.fused = pk_fused<LR, LER, GR>();
-    c.fwd_stack = pk_stack<LR, LER, GR, true>();
-    c.inv_stack = pk_stack<LR, LER, GR, false>();
-    c.fused_stack = pk_fused_stack<LR, LER, GR>();
-    return c;
-}
-
 std::vector<std::vector<NttConfig>> build_registry() {
-    std::vector<std::vector<NttConfig>> r(kMaxLogN + 1);
-    r[1] = {single<1, 1, 256>()};
-    r[2] = {single<2, 2, 256>()};
-    r[3] = {single<3, 3, 256>()};
-    r[4] = {single<4, 4, 256>()};
-    r[5] = {single<5, 4, 128>(), single<5, 3, 64>()};
-    r[6] = {single<6, 4, 64>(), single<6, 3, 32>()};
-    r[7] = {single<7, 4, 32>(), single<7, 3, 16>()};
-    r[8] = {single<8, 4, 16>(), single<8, 3, 8>()};
-    r[9] = {single<9, 4, 8>(), single<9, 3, 4>()};
-    r[10] = {single<10, 4, 4>(), single<10, 3, 2>()};
-    r[11] = {single<11, 4, 2>(), single<11, 3, 1>()};
-    // two-pass forms of 2^12 and 2^13: more, shorter workgroups, for single calls
-    r[12] = {single<12, 4, 1>(), single<12, 3, 1>(), two<6, 3, 8, 6, 3, 8>()};
-    r[13] = {single<13, 4, 1>(), single<13, 3, 1>(), two<6, 3, 16, 7, 3, 16>(), two<6, 3, 8, 7, 3, 8>()};
-    r[14] = {two<7, 4, 32, 7, 4, 32>(), two<7, 4, 16, 7, 4, 16>(), two<7, 3, 16, 7, 3, 16>(),
-             two<7, 3, 32, 7, 3, 16>()};
-    r[15] = {two<8, 4, 16, 7, 4, 32>(), two<7, 4, 32, 8, 4, 16>(), two<8, 3, 16, 7, 3, 16>(),
-             two<7, 3, 16, 8, 3, 8>()};
-    r[16] = {two<8, 4, 16, 8, 4, 16>(), two<8, 4, 8, 8, 4, 8>(), two<8, 3, 16, 8, 3, 8>(),
-             two<8, 3, 16, 8, 4, 8>(),  two<8, 4, 16, 8, 3, 8>(), two<8, 3, 8, 8, 3, 8>(),
-             two<8, 3, 16, 8, 3, 16>(), two<7, 3, 16, 9, 3, 4>(), two<9, 3, 8, 7, 3, 16>()};
-    r[17] = {two<9, 3, 8, 8, 4, 16>(), two<8, 4, 16, 9, 3, 8>(), two<9, 3, 8, 8, 3, 8>(),
-             two<8, 3, 16, 9, 3, 4>()};
+    Registry r(kMaxLogN + 1);
+    add_configs_1_13(r);
+    add_configs_14_15(r);
+    add_configs_16a(r);
+    add_configs_16b(r);
+    add_configs_17(r);
     return r;
 }
 

@@ -1,0 +1,13 @@
+// NTT configurations for 2^14 and 2^15 (see ntt_registry.hpp).
+#include "ntt_registry.hpp"
+
+namespace axhel_host {
+
+void add_configs_14_15(Registry &r) {
+    r[14] = {two<7, 4, 32, 7, 4, 32>(), two<7, 4, 16, 7, 4, 16>(), two<7, 3, 16, 7, 3, 16>(),
+             two<7, 3, 32, 7, 3, 16>()};
+    r[15] = {two<8, 4, 16, 7, 4, 32>(), two<7, 4, 32, 8, 4, 16>(), two<8, 3, 16, 7, 3, 16>(),
+             two<7, 3, 16, 8, 3, 8>()};
+}
+
+}  // namespace axhel_host

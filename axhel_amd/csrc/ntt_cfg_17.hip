@@ -1,0 +1,11 @@
+// NTT configurations for 2^17 (see ntt_registry.hpp).
+#include "ntt_registry.hpp"
+
+namespace axhel_host {
+
+void add_configs_17(Registry &r) {
+    r[17] = {two<9, 3, 8, 8, 4, 16>(), two<8, 4, 16, 9, 3, 8>(), two<9, 3, 8, 8, 3, 8>(),
+             two<8, 3, 16, 9, 3, 4>()};
+}
+
+}  // namespace axhel_host
