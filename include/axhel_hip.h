@@ -111,7 +111,8 @@ axhel_hip_status axhel_hip_ntt_plan_tables(const axhel_hip_plan *plan, uint64_t 
 
 /* NTT::ComputeForward: ntt.hpp:49 (ntt.cpp:173-186).  input_mod_factor is
    ignored as in the reference; output_mod_factor == 4 gives the lazy [0,4q)
-   output, anything else the normalized [0,q) output. */
+   output, anything else the normalized [0,q) output.  Operand words are in
+   [0,4q), the reference's domain (ntt-native.cpp:17). */
 axhel_hip_status axhel_hip_ntt_forward(const axhel_hip_plan *plan, uint64_t *result, const uint64_t *operand,
                                        uint64_t input_mod_factor, uint64_t output_mod_factor, void *stream);
 /* NTT::ComputeInverse: ntt.hpp:53 (ntt.cpp:192-204).  output_mod_factor == 2
