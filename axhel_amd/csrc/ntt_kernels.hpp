@@ -15,11 +15,11 @@
 // with the reference's twiddle, and butterflies on one bit run only after all
 // butterflies on the previous bit that feed them, so outputs are bit-identical
 // to the scalar loop, lazy bits included.  Only the order of independent
-// butterflies changes.  The one exception is opt-in (RLX = true, forward
-// kernels launched for a normalized output with every q < 2^61): the
+// butterflies changes.  The one exception is chosen by the host for forward
+// transforms with a normalized output and every q < 2^61 (RLX = true): the
 // conditional subtractions follow the relaxed schedule of fwd_bfly_relaxed,
 // whose intermediate words differ but whose normalized outputs, the unique
-// residues in [0, q), are the reference's.
+// residues in [0, q), are the reference's for inputs in [0, 4q).
 //
 // Mapping to the hardware.  A pass handles a contiguous range of LOG_G bit
 // positions.  Fixing every other index bit leaves a "group" of G = 2^LOG_G
@@ -27,9 +27,11 @@
 //   ROW pass: bits [0, LOG_G) -> the group is G contiguous words;
 //   COL pass: bits [logN-LOG_G, logN) -> the group is G words at stride
 //             2^(logN-LOG_G) (a column of the N/G x G row-major matrix).
-// N <= 2^13 runs as one ROW pass (the whole limb sits in LDS); larger N
-// runs a COL pass and a ROW pass (forward: COL then ROW; inverse: ROW then
-// COL), two HBM round trips in all.
+// Small N can run as one ROW pass (the whole limb sits in LDS; every
+// configuration of N <= 2^11, and the batch configurations of 2^12 and
+// 2^13); larger N, and the single-call shapes of 2^12 and 2^13, run a COL
+// pass and a ROW pass (forward: COL then ROW; inverse: ROW then COL), two
+// HBM round trips in all.
 //
 // Inside a group, T = G/E threads each hold E = 2^LOG_E words in registers.
 // The pass's bits are cut into chunks of LOG_E bits (the last may be
