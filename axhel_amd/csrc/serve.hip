@@ -478,7 +478,7 @@ axhel_hip_status axhel_hip_server_create(const axhel_hip_serve_config *config, a
     if (!s) return AXHEL_HIP_NO_MEMORY;
     if (config) s->cfg = *config;
     if (s->cfg.max_batch_limbs == 0) s->cfg.max_batch_limbs = 8192;
-    if (s->cfg.slots == 0) s->cfg.slots = 3;
+    if (s->cfg.slots == 0) s->cfg.slots = 4;  // one stream per hardware queue of a process (HIP's default 4)
     s->device = s->cfg.backend == AXHEL_HIP_BACKEND_DEVICE;
     if (s->device && axhel_hip_device_count() == 0) {
         delete s;

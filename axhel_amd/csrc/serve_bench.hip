@@ -260,7 +260,7 @@ std::string poisson(Bench &B, double rate, double seconds) {
 int main(int argc, char **argv) {
     double seconds = 5.0;
     std::vector<double> fractions = {0.5, 0.8};
-    uint32_t slots = 3;
+    uint32_t slots = 4;
     uint64_t seed = 0;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];

@@ -229,7 +229,7 @@ typedef enum axhel_hip_backend {
 typedef struct axhel_hip_serve_config {
     uint64_t memory_budget_bytes; /* device bytes for staging host operands; 0 = half the free HBM */
     uint32_t max_batch_limbs;     /* limbs packed into one device batch; 0 = 8192 */
-    uint32_t slots;               /* device batches in flight; 0 = 3 */
+    uint32_t slots;               /* device batches in flight; 0 = 4 */
     uint32_t backend;             /* axhel_hip_backend */
     uint32_t null_latency_us;     /* NULL backend: simulated batch duration */
     uint64_t fail_every_alloc;    /* fault injection: every k-th staging allocation fails; 0 = never */
