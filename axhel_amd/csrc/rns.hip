@@ -8,8 +8,9 @@
 // crosses PCIe, into a table buffer the context keeps per stream: calls on
 // one stream run in order, so the buffer is free again for the next call on
 // that stream, and calls on different streams never share one.  A call that
-// repeats the previous NTT call on its stream (same buffers, count and
-// direction) finds its table in place and skips the fill.  The context's
+// repeats the previous call on its stream (NTT: same buffers, count,
+// direction and stack layout; element-wise: same buffers, count and per-limb
+// scalars) finds its table in place and skips the fill.  The context's
 // mutex is held while a call enqueues its work, so two threads issuing on
 // one stream cannot interleave a fill with another call's passes.  The
 // transforms then run through the batched kernels, the ROW pass stacked over
@@ -17,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -57,6 +59,12 @@ struct axhel_hip_rns {
         uint64_t count;
         int inverse;
         uint32_t slots;  // stack slots per limb (0: no stack table)
+        // or the element-wise call whose table it holds (the op and factor
+        // are launch parameters, not table content)
+        bool elt_valid;
+        const void *e_r, *e_a, *e_b;
+        uint64_t e_count;
+        uint64_t e_s[AXHEL_HIP_RNS_MAX_LIMBS];
     };
     std::mutex mu;
     std::vector<StreamTable> tables;
@@ -160,8 +168,10 @@ hipError_t stream_table(axhel_hip_rns *ctx, hipStream_t st, size_t bytes, axhel_
     for (auto &e : ctx->tables)
         if (e.stream == st) t = &e;
     if (!t) {
-        ctx->tables.push_back({st, nullptr, 0, false, nullptr, nullptr, 0, 0, 0});
+        ctx->tables.emplace_back();
         t = &ctx->tables.back();
+        *t = axhel_hip_rns::StreamTable{};
+        t->stream = st;
     }
     if (t->bytes < bytes) {
         if (t->buf) {
@@ -172,6 +182,7 @@ hipError_t stream_table(axhel_hip_rns *ctx, hipStream_t st, size_t bytes, axhel_
             t->bytes = 0;
         }
         t->ntt_valid = false;
+        t->elt_valid = false;
         const size_t want = std::max(bytes, size_t(64) << 10);
         hipError_t e = hipMalloc(&t->buf, want);
         if (e != hipSuccess) return e;
@@ -295,6 +306,7 @@ axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const u
                       tab_buf->count == count && tab_buf->inverse == inverse && tab_buf->slots == slots_per_limb;
     if (!same) {
         tab_buf->ntt_valid = false;
+        tab_buf->elt_valid = false;  // the NTT table overwrites the buffer
         const uint64_t fill = std::max<uint64_t>(nj, (uint64_t)slots_per_limb * L);
         hipLaunchKernelGGL(rns_fill_ntt, dim3((unsigned)((fill + 255) / 256)), dim3(256), 0, st, jobs, stacks,
                            ctx->d_info, result, operand, cnt, L, ctx->n, inverse, slots_per_limb);
@@ -398,11 +410,28 @@ axhel_hip_status axhel_hip_rns_eltwise(axhel_hip_rns *ctx, int op, uint64_t *res
     std::lock_guard<std::mutex> lk(ctx->mu);
     axhel_hip_rns::StreamTable *tab_buf = nullptr;
     AXHEL_HIP_TRY(stream_table(ctx, st, nl * sizeof(EltLimbDev), &tab_buf));
-    tab_buf->ntt_valid = false;  // the element-wise table overwrites the buffer
     EltLimbDev *d = (EltLimbDev *)tab_buf->buf;
-    hipLaunchKernelGGL(rns_fill_elt, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st, d, ctx->d_info, result,
-                       operand1, b, (uint32_t)count, L, ctx->n, sc);
-    hipError_t e = hipGetLastError();
+    // the table depends on the buffers, the count and the per-limb scalars: a
+    // repeated call finds it in place
+    const bool same = tab_buf->elt_valid && tab_buf->e_r == result && tab_buf->e_a == operand1 &&
+                      tab_buf->e_b == b && tab_buf->e_count == count &&
+                      std::memcmp(tab_buf->e_s, sc.s, L * sizeof(uint64_t)) == 0;
+    hipError_t e = hipSuccess;
+    if (!same) {
+        tab_buf->ntt_valid = false;  // the element-wise table overwrites the buffer
+        tab_buf->elt_valid = false;
+        hipLaunchKernelGGL(rns_fill_elt, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st, d, ctx->d_info,
+                           result, operand1, b, (uint32_t)count, L, ctx->n, sc);
+        e = hipGetLastError();
+        if (e == hipSuccess) {
+            tab_buf->elt_valid = true;
+            tab_buf->e_r = result;
+            tab_buf->e_a = operand1;
+            tab_buf->e_b = b;
+            tab_buf->e_count = count;
+            std::memcpy(tab_buf->e_s, sc.s, L * sizeof(uint64_t));
+        }
+    }
     if (e == hipSuccess) e = launch_elt_batch(eop, f, d, (uint32_t)nl, ctx->n, st);
     AXHEL_HIP_TRY(e);
     return AXHEL_HIP_OK;

@@ -203,6 +203,67 @@ def test_rns_cached_tables_follow_every_change():
     ctx.close()
 
 
+def test_rns_cached_eltwise_tables_follow_every_change():
+    """An element-wise call that repeats the previous call on its stream (same
+    buffers, count and per-limb scalars) skips the fill; the op and factors
+    are not part of the table.  In sequence on one stream: repeated calls,
+    another op on the same buffers, vector-scalar calls whose scalars alone
+    change, an NTT call in between, a smaller count and another result
+    buffer; every result equals the per-limb calls."""
+    n, L, count = 1 << 12, 5, 9
+    torch, qs, ctx, a_np, a = setup(n, L, count, 41, 4)
+    rng = np.random.default_rng(42)
+    b = torch.from_numpy(np.stack([np.stack([params.uniform(rng, n, 4 * q) for q in qs])
+                                   for _ in range(count)]).view(np.int64).copy()).cuda()
+    qv = torch.tensor(qs, device="cuda").view(1, L, 1)
+    lo, lo_b = torch.remainder(a, qv), torch.remainder(b, qv)
+    s1 = [int(params.uniform(rng, 1, q)[0]) for q in qs]
+    s2 = [int(params.uniform(rng, 1, q)[0]) for q in qs]
+    s8 = [int(params.uniform(rng, 1, 8 * q)[0]) for q in qs]
+
+    def ref(plain, x, y, cnt=count):
+        out = torch.full_like(a, -1)
+        for p in range(cnt):
+            for l in range(L):
+                plain(out[p, l], x[p, l], None if y is None else y[p, l], qs[l], l)
+        torch.cuda.synchronize()
+        return out
+
+    mul = ref(lambda r, x, y, q, l: ax.EltwiseMulMod(r, x, y, n, q, 1), lo, lo_b)
+    add = ref(lambda r, x, y, q, l: ax.EltwiseAddMod(r, x, y, n, q), lo, lo_b)
+    add1 = ref(lambda r, x, y, q, l: ax.EltwiseAddMod(r, x, s1[l], n, q), lo, None)
+    add2 = ref(lambda r, x, y, q, l: ax.EltwiseAddMod(r, x, s2[l], n, q), lo, None)
+    sub2 = ref(lambda r, x, y, q, l: ax.EltwiseSubMod(r, x, s2[l], n, q), lo, None)
+    fma8 = ref(lambda r, x, y, q, l: ax.EltwiseFMAMod(r, x, s8[l], y, n, q, 8), a, b)
+    fma4 = ref(lambda r, x, y, q, l: ax.EltwiseFMAMod(r, x, s8[l], y, n, q, 4), lo, lo_b)
+    out = torch.empty_like(a)
+
+    def check(want, op, x, y, sc, fi, cnt=count, dst=out):
+        dst.fill_(-1)
+        ctx.eltwise(op, dst, x, y, sc, cnt, fi, 1)
+        torch.cuda.synchronize()
+        assert torch.equal(dst[:cnt], want[:cnt]) and bool((dst[cnt:] == -1).all())
+
+    for _ in range(3):  # identical calls: table reused
+        check(mul, ax.OP_MUL, lo, lo_b, None, 1)
+    check(add, ax.OP_ADD, lo, lo_b, None, 1)  # same table, another op
+    check(add1, ax.OP_ADD, lo, None, s1, 1)  # operand2 gone: refill
+    check(add2, ax.OP_ADD, lo, None, s2, 1)  # only the scalars differ
+    check(sub2, ax.OP_SUB, lo, None, s2, 1)
+    check(fma8, ax.OP_FMA, a, b, s8, 8)  # scalars stored reduced for F=8
+    check(fma4, ax.OP_FMA, lo, lo_b, s8, 4)  # same raw scalars, other operands and reduction
+    fwd = torch.empty_like(a)
+    ctx.ntt(fwd, lo, count, False, 1)  # the NTT table overwrites the buffer
+    check(fma4, ax.OP_FMA, lo, lo_b, s8, 4)
+    check(mul, ax.OP_MUL, lo, lo_b, None, 1, cnt=4)  # smaller count
+    other = torch.empty_like(a)
+    check(mul, ax.OP_MUL, lo, lo_b, None, 1, dst=other)  # another result buffer
+    check(mul, ax.OP_MUL, lo, lo_b, None, 1)
+    t = O.Tables(n, qs[3], params.find_primitive_2n_root(n, qs[3]))
+    assert np.array_equal(host(fwd[2, 3]), O.compute_forward(host(lo[2, 3]), t, 1, 1))
+    ctx.close()
+
+
 def test_rns_calls_refuse_graph_capture():
     """A captured call would depend on the context's per-stream table buffer,
     which later calls overwrite: calls on a capturing stream are refused
