@@ -30,15 +30,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n, primes, psis, seconds):
     """The oracle's forward NTT (C restatement of the reference Native path)
     on host threads, one independent limb per thread: limbs/s over a bounded
-    sample of the same workload."""
+    sample of the same workload.  A shorter single-thread sample first gives
+    the one-core per-call latency."""
     from oracle import oracle as O
     threads = min(16, os.cpu_count() or 1)
     tabs = [O.Tables(n, q, psi) for q, psi in zip(primes, psis)]
     rng = np.random.default_rng(7)
     bufs = [rng.integers(0, primes[i % len(primes)], size=n, dtype=np.uint64) for i in range(threads)]
+    one_s = min(3.0, seconds / 4)
+    c1, t1 = 0, time.perf_counter()
+    while time.perf_counter() - t1 < one_s:
+        O.compute_forward_inplace(bufs[0], tabs[c1 % len(tabs)], 1)
+        c1 += 1
+    e1 = time.perf_counter() - t1
+    one_core = {"value": c1 / e1, "unit": "limbs/s", "latency_us": e1 / c1 * 1e6,
+                "sample": f"{c1} forward NTTs at N={n} on one thread over {e1:.1f} s"}
     counts = [0] * threads
     stop = time.perf_counter() + seconds
 
@@ -60,7 +80,8 @@ def cpu_baseline(n, primes, psis, seconds):
     el = time.perf_counter() - t0
     total = sum(counts)
     return {"value": total / el, "unit": "limbs/s", "cores": threads, "kind": "port",
-            "sample": f"{total} forward NTTs at N={n} over {el:.1f} s, one limb per thread"}
+            "sample": f"{total} forward NTTs at N={n} over {el:.1f} s, one limb per thread",
+            "host_cpu": _cpu_model(), "host_cpus_visible": os.cpu_count(), "one_core": one_core}
 
 
 def eltwise_leg(ax, torch, src, dst, q, reps):
