@@ -12,7 +12,7 @@ import pytest
 import axhel_amd as ax
 from axhel_amd import params
 from oracle import oracle as O
-from tests.helpers import NTT_DEGREES, MODULUS_BITS, SEED, dev, empty_dev, host, ntt_params
+from tests.helpers import NTT_DEGREES, MODULUS_BITS, SEED, dev, empty_dev, host, ntt_params, torch_mod
 
 pytestmark = pytest.mark.gpu
 
@@ -278,3 +278,64 @@ def test_host_pointer_calls_reuse_pooled_staging_buffers():
     for th in ths:
         th.join()
     assert not errors, errors
+
+
+@pytest.mark.parametrize("logn", [1, 4, 10, 13, 16, 17])
+def test_plain_calls_on_8_byte_aligned_buffers(logn):
+    """Only 8-byte alignment is guaranteed (the reference's UnalignedOffsets
+    cases, test_eltwise.cpp:518-561): operands and results one word into an
+    allocation take the 8-byte access path of every compiled configuration,
+    in place and out of place, and equal the oracle."""
+    torch = torch_mod()
+    n = 1 << logn
+    q, psi = ntt_params(n, 55)
+    ntt = ax.NTT(n, q, psi)
+    t = O.Tables(n, q, psi)
+    a4 = O.MT19937_64(SEED + logn).random_vector(n, 4 * q)
+    a2 = O.MT19937_64(SEED + logn + 1).random_vector(n, 2 * q)
+    base = torch.zeros(2 * n + 2, dtype=torch.int64, device="cuda")
+    src, dst = base[1:n + 1], base[n + 1:2 * n + 1]
+    assert src.data_ptr() % 16 == 8 and dst.data_ptr() % 16 == 8
+    try:
+        for ci in range(ax.config_count(n)):
+            ax.config_select(n, ci)
+            for inverse, a, f in ((False, a4, 4), (True, a2, 2)):
+                call = ntt.ComputeInverse if inverse else ntt.ComputeForward
+                want_f = O.compute_inverse if inverse else O.compute_forward
+                for of in (f, 1):
+                    src.copy_(torch.from_numpy(a.view(np.int64)))
+                    call(dst, src, f, of)
+                    assert np.array_equal(host(dst), want_f(a, t, f, of)), (ci, inverse, of, "out of place")
+                    call(src, src, f, of)
+                    assert np.array_equal(host(src), want_f(a, t, f, of)), (ci, inverse, of, "in place")
+        assert bool((base[0] == 0).item()) and bool((base[-1] == 0).item())  # neighbours untouched
+    finally:
+        ax.config_select(n, -1)
+
+
+def test_rns_calls_on_8_byte_aligned_buffers():
+    """An RNS context's whole-polynomial calls on arrays one word into an
+    allocation equal the same calls on 16-byte-aligned arrays."""
+    torch = torch_mod()
+    n, L, count = 1 << 14, 3, 4
+    qs = params.rns_primes(n, L)
+    ctx = ax.RNS(n, qs)
+    qv = torch.tensor(qs, device="cuda").view(1, L, 1)
+    words = count * L * n
+    src = torch.remainder(torch.randint(0, 1 << 62, (count, L, n), dtype=torch.int64, device="cuda"), qv)
+    base = torch.zeros(2 * words + 1, dtype=torch.int64, device="cuda")
+    usrc, udst = base[1:words + 1].view(count, L, n), base[words + 1:].view(count, L, n)
+    usrc.copy_(src)
+    assert usrc.data_ptr() % 16 == 8
+    for inverse in (False, True):
+        want = torch.empty_like(src)
+        ctx.ntt(want, src, count, inverse, 1)
+        ctx.ntt(udst, usrc, count, inverse, 1)
+        torch.cuda.synchronize()
+        assert torch.equal(udst, want), inverse
+    want = torch.empty_like(src)
+    ctx.eltwise(ax.OP_MUL, want, src, src, None, count, 1, 1)
+    ctx.eltwise(ax.OP_MUL, udst, usrc, usrc, None, count, 1, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(udst, want) and bool((base[0] == 0).item())
+    ctx.close()
