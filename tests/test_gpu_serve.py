@@ -154,3 +154,33 @@ def test_host_buffers_are_staged_through_the_pool():
     assert np.array_equal(x, a)
     assert server.stats()["bytes_in_use"] == 0
     server.close()
+
+
+def test_requests_on_8_byte_aligned_buffers():
+    """Operands and results one word into an allocation (only 8-byte
+    alignment is guaranteed) take the servers' 8-byte access path; every kind
+    of request still equals the oracle, in place and out of place."""
+    torch = torch_mod()
+    rng = np.random.default_rng(91)
+    server = ax.Server(slots=2)
+    reqs = make_requests(server, rng, 12)
+    for i, R in enumerate(reqs):
+        if R["kind"] == "fwd":
+            R["lazy"] = bool(i % 4 == 0)
+            R["want"] = oracle_fwd(R, R["lazy"])
+        words = R["n"] * R["limbs"]
+        base = torch.zeros(3 * words + 1, dtype=torch.int64, device="cuda")
+        da, db, out = base[1:words + 1], base[words + 1:2 * words + 1], base[2 * words + 1:]
+        da.copy_(dev(R["a"]))
+        db.copy_(dev(R["b"]))
+        assert da.data_ptr() % 16 == 8 and out.data_ptr() % 16 == 8
+        inplace = i % 3 == 2
+        dst = da if inplace else out
+        if R["kind"] == "fwd":
+            st, t = submit_fwd(server, R, dst, da, R["lazy"])
+        else:
+            st, t = submit(server, R, dst, da, db)
+        assert st == ax.OK and server.wait(t) == ax.OK, (i, R["kind"])
+        assert np.array_equal(host(dst), R["want"]), (i, R["kind"], R["n"], inplace)
+        assert bool((base[0] == 0).item())
+    server.close()
