@@ -115,3 +115,27 @@ def test_tuning_table_sets_batch_and_single_call_configurations():
     # a degree the table does not list: the first configuration for both
     assert 9 not in table
     assert axhel_amd.config_selected(1 << 9) == 0 and axhel_amd.config_single(1 << 9) == 0
+
+
+def test_missing_library_fails_loudly():
+    """Without the built HIP library every entry point raises ImportError
+    before computing anything: there is no CPU fallback.  Run in a child
+    process so this process keeps its loaded library."""
+    code = "\n".join([
+        "import numpy as np, axhel_amd as ax",
+        "ax.LIB_PATH, ax._lib = '/nonexistent/libaxhel_hip.so', None",
+        "a = np.arange(8, dtype=np.uint64); r = np.zeros_like(a)",
+        "calls = [lambda: ax.NTT(8, 17, 3), lambda: ax.EltwiseAddMod(r, a, a, 8, 17),",
+        "         lambda: ax.EltwiseMulMod(r, a, a, 8, 17, 1), lambda: ax.RNS(8, [17])]",
+        "for c in calls:",
+        "    try:",
+        "        c()",
+        "    except ImportError as e:",
+        "        assert 'is not built' in str(e)",
+        "    else:",
+        "        raise SystemExit('a call ran without the library')",
+        "assert r.tolist() == [0] * 8",
+        "print('all raised')",
+    ])
+    out = subprocess.run(["python", "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and "all raised" in out.stdout, out.stdout + out.stderr
