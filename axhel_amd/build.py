@@ -75,6 +75,7 @@ def build(verbose=False, jobs=None):
     # serving benchmark driver (host program over the C ABI)
     sb = os.path.join(LIBDIR, "axhel_serve_bench")
     _run([HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-Wno-unused-command-line-argument",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "serve_bench.hip"),
           "-L", LIBDIR, "-laxhel_hip", "-lpthread", "-Wl,-rpath,$ORIGIN", "-o", sb + ".tmp"], verbose)
     os.replace(sb + ".tmp", sb)
