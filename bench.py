@@ -378,8 +378,14 @@ def main():
     alg_bytes = 16.0 * n * limbs
     achieved_bfly = bfly / (kdur[dom] * 1e-3) / 1e9
     achieved_gbs = alg_bytes / (kdur[dom] * 1e-3) / 1e9
-    peak_bfly, _ = ax.butterfly_peak()
-    peak_bfly /= 1e9
+    # register-resident butterfly peak: the microbenchmark back to back for
+    # ~0.3 s, median of the second half.  The first runs after the host-side
+    # gap start at a lower clock and read ~10 % low; the first run is
+    # reported beside the peak.
+    runs, t_end = [], time.perf_counter() + 0.3
+    while time.perf_counter() < t_end or len(runs) < 8:
+        runs.append(ax.butterfly_peak()[0] / 1e9)
+    peak_bfly = float(np.median(runs[len(runs) // 2:]))
     cfg = ax.config_get(n, ax.config_selected(n))
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "ntt_n65536_64x24_hbm_bytes.txt")
@@ -402,7 +408,8 @@ def main():
                 ach = insts / (kdur[dom] * 1e-3) / 1e9
                 valu_issue = {"achieved": ach, "peak": VALU_ISSUE_PEAK, "unit": "G wave64 VALU instr/s",
                               "frac": ach / VALU_ISSUE_PEAK, "insts_per_launch": insts}
-    log(f"dominant pass {dom}: {achieved_bfly:.1f} Gbfly/s of {peak_bfly:.1f} peak; {achieved_gbs:.0f} GB/s"
+    log(f"dominant pass {dom}: {achieved_bfly:.1f} Gbfly/s of {peak_bfly:.1f} peak "
+        f"(first of {len(runs)} runs {runs[0]:.1f}); {achieved_gbs:.0f} GB/s"
         + (f"; VALU issue {valu_issue['frac']:.2f} of peak" if valu_issue else ""))
 
     # plain per-call path: one synchronous ComputeForward per limb
@@ -481,7 +488,9 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved_bfly, "peak": peak_bfly, "unit": "Gbfly/s",
                      "frac": achieved_bfly / peak_bfly, "traffic": traffic,
                      "kernel": f"ntt pass {dom} of {kcount} ({kdur[dom]:.4f} ms)",
-                     "peak_source": "register-resident Harvey butterflies, all CUs (axhel_hip_ubench_butterfly_peak)",
+                     "peak_source": "register-resident Harvey butterflies, all CUs (axhel_hip_ubench_butterfly_peak), "
+                                    f"median of the last {len(runs) - len(runs) // 2} of {len(runs)} back-to-back runs",
+                     "peak_first_run": runs[0],
                      "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved_gbs / HBM_PEAK_GBS},
                      "valu_issue": valu_issue},
