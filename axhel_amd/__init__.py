@@ -63,6 +63,25 @@ class NttConfig(ctypes.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class NttKernelInfo(ctypes.Structure):
+    """axhel_hip_ntt_kernel_info (include/axhel_hip.h)."""
+    _fields_ = [("degree", _u64), ("limbs", ctypes.c_uint32), ("pass_", ctypes.c_uint32),
+                ("passes", ctypes.c_uint32), ("log_bits", ctypes.c_uint32), ("inverse", ctypes.c_int32),
+                ("normalize", ctypes.c_int32), ("relaxed", ctypes.c_int32), ("stacked", ctypes.c_int32),
+                ("schedule", ctypes.c_int32), ("reserved", ctypes.c_int32), ("butterflies", _u64), ("bytes", _u64)]
+
+    def as_dict(self):
+        d = {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "reserved"}
+        d["pass"] = d.pop("pass_")
+        d["schedule_name"] = SCHEDULE_NAMES[d["schedule"]]
+        return d
+
+
+# butterfly schedules of the pass kernels (include/axhel_hip_ubench.h)
+SCHEDULE_NAMES = ["fwd_exact", "fwd_relaxed", "fwd_relaxed_last_normalized", "inv", "inv_final",
+                  "inv_final_normalized"]
+
+
 class PolymulJob(ctypes.Structure):
     """axhel_hip_polymul_job (include/axhel_hip.h)."""
     _fields_ = [("plan", _p), ("result", _p), ("a", _p), ("b", _p), ("addend", _p), ("scalar", _u64),
@@ -72,7 +91,8 @@ class PolymulJob(ctypes.Structure):
 class ServeConfig(ctypes.Structure):
     """axhel_hip_serve_config (include/axhel_hip.h)."""
     _fields_ = [("memory_budget_bytes", _u64), ("max_batch_limbs", ctypes.c_uint32), ("slots", ctypes.c_uint32),
-                ("backend", ctypes.c_uint32), ("null_latency_us", ctypes.c_uint32), ("fail_every_alloc", _u64)]
+                ("backend", ctypes.c_uint32), ("null_latency_us", ctypes.c_uint32), ("fail_every_alloc", _u64),
+                ("fail_every_batch", _u64)]
 
 
 class Request(ctypes.Structure):
@@ -87,7 +107,8 @@ class ServeStats(ctypes.Structure):
     _fields_ = [(k, _u64) for k in ("submitted", "completed", "rejected", "timed_out", "cancelled", "failed",
                                     "limbs_completed", "batches", "bytes_in_use", "peak_bytes_in_use")] + \
                [(k, ctypes.c_double) for k in ("p50_us", "p99_us", "mean_us", "max_us",
-                                              "first_arrival_to_last_completion_s")]
+                                              "first_arrival_to_last_completion_s")] + \
+               [("descriptor_bytes", _u64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -125,6 +146,7 @@ SIGNATURES = {
     "axhel_hip_ntt_batch_kernel_count": (ctypes.c_int, [_p]),
     "axhel_hip_ntt_batch_launch_kernel": (_st, [_p, ctypes.c_int, _p]),
     "axhel_hip_ntt_batch_run": (_st, [ctypes.POINTER(NttJob), ctypes.c_size_t, _p]),
+    "axhel_hip_ntt_batch_kernel_info": (_st, [_p, ctypes.c_int, ctypes.POINTER(NttKernelInfo)]),
     "axhel_hip_ntt_config_count": (ctypes.c_int, [_u64]),
     "axhel_hip_ntt_batch_stacked_groups": (ctypes.c_int, [_p]),
     "axhel_hip_ntt_config_get": (_st, [_u64, ctypes.c_int, ctypes.POINTER(NttConfig)]),
@@ -150,6 +172,8 @@ SIGNATURES = {
     "axhel_hip_server_stats": (_st, [_p, ctypes.POINTER(ServeStats)]),
     "axhel_hip_server_reset_stats": (_st, [_p]),
     "axhel_hip_ubench_butterfly_peak": (_st, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "axhel_hip_ubench_schedule_peak": (_st, [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_double)]),
 }
 
 
@@ -365,6 +389,13 @@ class Batch:
     def launch_kernel(self, index, stream=None):
         _check(_load().axhel_hip_ntt_batch_launch_kernel(self._h, index, _stream_ptr(stream)), "launch_kernel")
 
+    def kernel_info(self, index):
+        """What the index-th launch of a replay computes: degree, limbs,
+        pass, butterflies, algorithmic bytes and butterfly schedule."""
+        info = NttKernelInfo()
+        _check(_load().axhel_hip_ntt_batch_kernel_info(self._h, index, ctypes.byref(info)), "kernel_info")
+        return info.as_dict()
+
     def stacked_groups(self):
         """Groups whose ROW pass runs over stacks of same-plan jobs."""
         return int(_load().axhel_hip_ntt_batch_stacked_groups(self._h))
@@ -381,10 +412,13 @@ class Batch:
             pass
 
 
-def butterfly_peak():
-    """Device-wide rate of register-resident forward Harvey butterflies."""
+def butterfly_peak(schedule=0):
+    """Device-wide rate of register-resident butterflies of one pass
+    schedule (SCHEDULE_NAMES; 0 = the exact forward butterfly), and the
+    timed launch's duration in seconds."""
     r, s = ctypes.c_double(), ctypes.c_double()
-    _check(_load().axhel_hip_ubench_butterfly_peak(ctypes.byref(r), ctypes.byref(s)), "butterfly_peak")
+    _check(_load().axhel_hip_ubench_schedule_peak(int(schedule), ctypes.byref(r), ctypes.byref(s)),
+           "butterfly_peak")
     return r.value, s.value
 
 
@@ -482,8 +516,9 @@ class Server:
     returns the final status code (OK, TIMEOUT, CANCELLED, NO_MEMORY, ...)."""
 
     def __init__(self, memory_budget_bytes=0, max_batch_limbs=0, slots=0, backend=BACKEND_DEVICE,
-                 null_latency_us=0, fail_every_alloc=0):
-        cfg = ServeConfig(memory_budget_bytes, max_batch_limbs, slots, backend, null_latency_us, fail_every_alloc)
+                 null_latency_us=0, fail_every_alloc=0, fail_every_batch=0):
+        cfg = ServeConfig(memory_budget_bytes, max_batch_limbs, slots, backend, null_latency_us, fail_every_alloc,
+                          fail_every_batch)
         h = _p()
         _check(_load().axhel_hip_server_create(ctypes.byref(cfg), ctypes.byref(h)), "server_create")
         self._h = h

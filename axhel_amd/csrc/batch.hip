@@ -11,11 +11,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <new>
 #include <tuple>
 #include <vector>
 
+#include "../../include/axhel_hip_ubench.h"
 #include "ntt_host.hpp"
 
 using axhel_dev::NttJobDev;
@@ -252,6 +254,48 @@ axhel_hip_status axhel_hip_ntt_batch_launch_kernel(axhel_hip_batch *batch, int i
             return AXHEL_HIP_OK;
         }
         index -= cfg.npass;
+    }
+    return AXHEL_HIP_INVALID_ARGUMENT;
+}
+
+axhel_hip_status axhel_hip_ntt_batch_kernel_info(const axhel_hip_batch *batch, int index,
+                                                 axhel_hip_ntt_kernel_info *info) {
+    if (!batch || !info || index < 0) return AXHEL_HIP_INVALID_ARGUMENT;
+    for (const auto &g : batch->groups) {
+        const NttConfig &cfg = config_for_launch(g.logn, g.count);
+        if (index >= cfg.npass) {
+            index -= cfg.npass;
+            continue;
+        }
+        // the choices launch_ntt makes for this pass
+        const int p = index;
+        const bool last = p == cfg.npass - 1;
+        const bool row_pass = cfg.npass == 2 && p == (g.inverse ? 0 : 1);
+        const PassKernel &ks = g.inverse ? cfg.inv_stack : cfg.fwd_stack;
+        const bool stacked = row_pass && g.stack_gpw && ks.fn && ks.gpw == g.stack_gpw;
+        const PassKernel &k = stacked ? ks : (g.inverse ? cfg.inv[p] : cfg.fwd[p]);
+        const bool relaxed = !g.inverse && forward_relaxed(cfg, g.normalize, g.relax_ok);
+        std::memset(info, 0, sizeof(*info));
+        info->degree = 1ull << g.logn;
+        info->limbs = g.count;
+        info->pass = (uint32_t)p;
+        info->passes = (uint32_t)cfg.npass;
+        info->log_bits = (uint32_t)k.log_g;
+        info->inverse = g.inverse;
+        info->normalize = last && g.normalize;
+        info->relaxed = relaxed;
+        info->stacked = stacked;
+        if (!g.inverse)
+            info->schedule = !relaxed ? AXHEL_HIP_UBENCH_FWD_EXACT
+                             : last   ? AXHEL_HIP_UBENCH_FWD_RELAXED_LAST
+                                      : AXHEL_HIP_UBENCH_FWD_RELAXED;
+        else
+            info->schedule = !last         ? AXHEL_HIP_UBENCH_INV
+                             : g.normalize ? AXHEL_HIP_UBENCH_INV_FINAL_NORM
+                                           : AXHEL_HIP_UBENCH_INV_FINAL;
+        info->butterflies = (info->degree / 2) * info->log_bits * g.count;
+        info->bytes = 16ull * info->degree * g.count;
+        return AXHEL_HIP_OK;
     }
     return AXHEL_HIP_INVALID_ARGUMENT;
 }

@@ -36,6 +36,10 @@ struct axhel_hip_polymul {
         size_t off_fstack;
         uint32_t fstacks;
         int fstack_gpw;
+        // the configuration the descriptors were built for (their a1/b1
+        // inputs depend on its pass count): launches always use it, whatever
+        // is selected for the degree later
+        int cfg_index;
     };
     std::vector<Group> groups;
     char *d_desc = nullptr;
@@ -84,10 +88,12 @@ bool stack_poly_jobs(const std::vector<PolyJobDev> &jobs, int gpw, std::vector<P
     return true;
 }
 
+const NttConfig &group_config(const axhel_hip_polymul::Group &g) { return configs_for(g.logn)[g.cfg_index]; }
+
 hipError_t issue(axhel_hip_polymul *pm, hipStream_t st) {
     static const NttJobDev none{};
     for (const auto &g : pm->groups) {
-        const NttConfig &cfg = config_for(g.logn);
+        const NttConfig &cfg = group_config(g);
         hipError_t e = prepare_config(cfg);
         if (e != hipSuccess) return e;
         const NttJobDev *col = (const NttJobDev *)(pm->d_desc + g.off_col);
@@ -174,6 +180,8 @@ axhel_hip_status axhel_hip_polymul_create(const axhel_hip_polymul_job *jobs, siz
         });
         axhel_hip_polymul::Group g{};
         g.logn = kv.first.first;
+        const NttConfig &cfg = config_for(g.logn);
+        g.cfg_index = (int)(&cfg - configs_for(g.logn).data());
         g.count = (uint32_t)idx.size();
         g.has_addend = kv.first.second != 0;
         g.vec16 = true;
@@ -194,8 +202,8 @@ axhel_hip_status axhel_hip_polymul_create(const axhel_hip_polymul_job *jobs, siz
             const auto &j = jobs[i];
             inv.push_back(make_job(j.plan, j.result, j.result, true));
             PolyJobDev d{};
-            d.a1 = config_for(g.logn).npass == 2 ? j.result : j.a;
-            d.b1 = config_for(g.logn).npass == 2 ? j.scratch : j.b;
+            d.a1 = cfg.npass == 2 ? j.result : j.a;
+            d.b1 = cfg.npass == 2 ? j.scratch : j.b;
             d.c = j.addend;
             d.out = j.result;
             d.twf = j.plan->d_fwd;
@@ -230,7 +238,6 @@ axhel_hip_status axhel_hip_polymul_create(const axhel_hip_polymul_job *jobs, siz
         g.off_fstack = 0;
         g.fstacks = 0;
         g.fstack_gpw = 0;
-        const NttConfig &cfg = config_for(g.logn);
         std::vector<PolyJobDev> fst;
         if (cfg.npass == 2 && cfg.fused_stack.fn && stack_poly_jobs(fz, cfg.fused_stack.gpw, fst, g.fstacks)) {
             g.off_fstack = put(buf, fst);
@@ -263,7 +270,7 @@ int axhel_hip_polymul_stacked_groups(const axhel_hip_polymul *pm) {
     if (!pm) return 0;
     int k = 0;
     for (const auto &g : pm->groups) {
-        const NttConfig &cfg = config_for(g.logn);
+        const NttConfig &cfg = group_config(g);
         k += pm->fused && g.fstack_gpw != 0 && cfg.fused_stack.fn && cfg.fused_stack.gpw == g.fstack_gpw;
     }
     return k;

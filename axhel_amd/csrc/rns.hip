@@ -210,6 +210,46 @@ axhel_hip_status refuse_capture(hipStream_t st) {
     return AXHEL_HIP_OK;
 }
 
+// Stack layout of a transform call over count x L limbs: slots per limb of
+// the stacked ROW pass (0: no stack table), the same rule as the batches
+// (padding each limb's polynomials to whole stacks leaves at most one slot
+// in eight empty).
+uint32_t stack_slots_per_limb(const NttConfig &cfg, uint32_t cnt, uint32_t L) {
+    const int gpw = stack_gpw_of(cfg);
+    if (!gpw) return 0;
+    const uint64_t nj = (uint64_t)cnt * L;
+    const uint64_t spl = (cnt + (uint64_t)gpw - 1) / gpw * gpw;
+    return (spl * L - nj) * 8 <= spl * L ? (uint32_t)spl : 0;
+}
+
+// Fill, then the passes, of one transform call whose tables go to `jobs`
+// (nj entries, then the stack table when slots_per_limb != 0).
+hipError_t enqueue_rns_ntt(const axhel_hip_rns *ctx, const NttConfig &cfg, NttJobDev *jobs, bool fill,
+                           uint64_t *result, const uint64_t *operand, uint32_t cnt, int inverse, bool normalize,
+                           uint32_t slots_per_limb, hipStream_t st) {
+    const uint32_t L = ctx->limbs;
+    const uint64_t nj = (uint64_t)cnt * L;
+    NttJobDev *stacks = slots_per_limb ? jobs + nj : nullptr;
+    if (fill) {
+        const uint64_t n_fill = std::max<uint64_t>(nj, (uint64_t)slots_per_limb * L);
+        hipLaunchKernelGGL(rns_fill_ntt, dim3((unsigned)((n_fill + 255) / 256)), dim3(256), 0, st, jobs, stacks,
+                           ctx->d_info, result, operand, cnt, L, ctx->n, inverse, slots_per_limb);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    static const NttJobDev none{};
+    StackTable tab;
+    if (stacks) {
+        const int gpw = stack_gpw_of(cfg);
+        tab.jobs = stacks;
+        tab.count = slots_per_limb * L / gpw;
+        tab.gpw = gpw;
+    }
+    const bool vec16 = aligned16(result) && aligned16(operand);
+    return launch_ntt(cfg, ctx->logn, inverse != 0, normalize, jobs, none, (uint32_t)nj, st, -1, vec16,
+                      stacks ? &tab : nullptr, ctx->relax_ok);
+}
+
 }  // namespace
 
 extern "C" {
@@ -244,6 +284,8 @@ axhel_hip_status axhel_hip_rns_create(uint64_t degree, const uint64_t *moduli, c
     if (st == AXHEL_HIP_OK) {
         hipError_t e = hipMalloc(&c->d_info, limbs * sizeof(RnsLimbDev));
         if (e == hipSuccess) e = hipMemcpy(c->d_info, info.data(), limbs * sizeof(RnsLimbDev), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = prepare_config(config_for_single(c->logn));
+        if (e == hipSuccess) e = prepare_config(config_for(c->logn));
         if (e != hipSuccess) {
             set_last_error("rns create", e);
             st = e == hipErrorOutOfMemory ? AXHEL_HIP_NO_MEMORY : AXHEL_HIP_HIP_ERROR;
@@ -284,21 +326,11 @@ axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const u
     const uint64_t nj = (uint64_t)cnt * L;
     const NttConfig &cfg = config_for_launch(ctx->logn, nj);
     AXHEL_HIP_TRY(prepare_config(cfg));
-    // stacked ROW pass when padding each limb's polynomials to whole stacks
-    // leaves at most one slot in eight empty (the batch rule)
-    const int gpw = stack_gpw_of(cfg);
-    uint32_t slots_per_limb = 0;
-    if (gpw) {
-        const uint64_t spl = (cnt + (uint64_t)gpw - 1) / gpw * gpw;
-        if ((spl * L - nj) * 8 <= spl * L) slots_per_limb = (uint32_t)spl;
-    }
+    const uint32_t slots_per_limb = stack_slots_per_limb(cfg, cnt, L);
     const uint64_t entries = nj + (uint64_t)slots_per_limb * L;
     std::lock_guard<std::mutex> lk(ctx->mu);
     axhel_hip_rns::StreamTable *tab_buf = nullptr;
     AXHEL_HIP_TRY(stream_table(ctx, st, entries * sizeof(NttJobDev), &tab_buf));
-    NttJobDev *jobs = (NttJobDev *)tab_buf->buf;
-    NttJobDev *stacks = slots_per_limb ? jobs + nj : nullptr;
-    hipError_t e = hipSuccess;
     // the table depends on the buffers, the count, the direction and the
     // stack layout (slots per limb, from the configuration's group count): a
     // repeated call finds it in place
@@ -307,30 +339,16 @@ axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const u
     if (!same) {
         tab_buf->ntt_valid = false;
         tab_buf->elt_valid = false;  // the NTT table overwrites the buffer
-        const uint64_t fill = std::max<uint64_t>(nj, (uint64_t)slots_per_limb * L);
-        hipLaunchKernelGGL(rns_fill_ntt, dim3((unsigned)((fill + 255) / 256)), dim3(256), 0, st, jobs, stacks,
-                           ctx->d_info, result, operand, cnt, L, ctx->n, inverse, slots_per_limb);
-        e = hipGetLastError();
-        if (e == hipSuccess) {
-            tab_buf->ntt_valid = true;
-            tab_buf->result = result;
-            tab_buf->operand = operand;
-            tab_buf->count = count;
-            tab_buf->inverse = inverse;
-            tab_buf->slots = slots_per_limb;
-        }
     }
-    if (e == hipSuccess) {
-        static const NttJobDev none{};
-        StackTable tab;
-        if (stacks) {
-            tab.jobs = stacks;
-            tab.count = slots_per_limb * L / gpw;
-            tab.gpw = gpw;
-        }
-        const bool vec16 = aligned16(result) && aligned16(operand);
-        e = launch_ntt(cfg, ctx->logn, inverse != 0, normalize, jobs, none, (uint32_t)nj, st, -1, vec16,
-                       stacks ? &tab : nullptr, ctx->relax_ok);
+    const hipError_t e = enqueue_rns_ntt(ctx, cfg, (NttJobDev *)tab_buf->buf, !same, result, operand, cnt, inverse,
+                                         normalize, slots_per_limb, st);
+    if (e == hipSuccess && !same) {
+        tab_buf->ntt_valid = true;
+        tab_buf->result = result;
+        tab_buf->operand = operand;
+        tab_buf->count = count;
+        tab_buf->inverse = inverse;
+        tab_buf->slots = slots_per_limb;
     }
     AXHEL_HIP_TRY(e);
     return AXHEL_HIP_OK;

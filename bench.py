@@ -54,14 +54,26 @@ def cpu_baseline(n, primes, psis, seconds):
     tabs = [O.Tables(n, q, psi) for q, psi in zip(primes, psis)]
     rng = np.random.default_rng(7)
     bufs = [rng.integers(0, primes[i % len(primes)], size=n, dtype=np.uint64) for i in range(threads)]
+    # one core, the reference benchmark's method (benchmark/src/axhel_benchmark.cpp:751-795):
+    # 10 warm-up calls, then individually timed calls (at least 100),
+    # Tukey-IQR filtered median and mean; tables built outside, in place
+    for i in range(10):
+        O.compute_forward_inplace(bufs[0], tabs[i % len(tabs)], 1)
     one_s = min(3.0, seconds / 4)
-    c1, t1 = 0, time.perf_counter()
-    while time.perf_counter() - t1 < one_s:
-        O.compute_forward_inplace(bufs[0], tabs[c1 % len(tabs)], 1)
-        c1 += 1
-    e1 = time.perf_counter() - t1
-    one_core = {"value": c1 / e1, "unit": "limbs/s", "latency_us": e1 / c1 * 1e6,
-                "sample": f"{c1} forward NTTs at N={n} on one thread over {e1:.1f} s"}
+    samples, t1 = [], time.perf_counter()
+    while len(samples) < 100 or time.perf_counter() - t1 < one_s:
+        t = tabs[len(samples) % len(tabs)]
+        a0 = time.perf_counter()
+        O.compute_forward_inplace(bufs[0], t, 1)
+        samples.append(time.perf_counter() - a0)
+    x = np.array(samples)
+    q1, q3 = np.percentile(x, [25, 75])
+    kept = x[(x >= q1 - 1.5 * (q3 - q1)) & (x <= q3 + 1.5 * (q3 - q1))]
+    med = float(np.median(kept))
+    one_core = {"value": 1.0 / med, "unit": "limbs/s", "latency_us": med * 1e6,
+                "latency_mean_us": float(kept.mean()) * 1e6,
+                "sample": f"{len(x)} forward NTTs at N={n} on one thread after 10 warm-up calls; "
+                          f"Tukey-IQR median of {len(kept)} kept samples"}
     counts = [0] * threads
     stop = time.perf_counter() + seconds
 
@@ -123,7 +135,7 @@ def eltwise_leg(ax, torch, src, dst, q, reps):
     return out
 
 
-def polymul_leg(ax, torch, params, polys=128, limbs=32, logn=15, reps=20, warm=10):
+def polymul_leg(ax, torch, params, polys=128, limbs=32, logn=15, reps=20, warm=10, peaks=None):
     """Fused RNS polynomial product, 128 polys x 32 limbs x N=2^15: NTT(a),
     NTT(b), MulMod, FMAMod(scalar, addend), InverseNTT per limb; fused
     pipeline vs the same five calls as batched kernels."""
@@ -161,10 +173,33 @@ def polymul_leg(ax, torch, params, polys=128, limbs=32, logn=15, reps=20, warm=1
         outs[key] = out.clone()
         pm.close()
     res["fused_equals_unfused"] = bool(torch.equal(outs["fused"], outs["unfused"]))
+    if peaks:
+        # per product: NTT(a), NTT(b) (relaxed forward: COL stages + ROW stages
+        # with the final stage and normalization) and one normalized inverse
+        cfg = ax.config_get(n, ax.config_selected(n))
+        col = cfg["log_col"]
+        row = logn - col
+        half = n // 2
+        stages = {1: 2 * col, 2: 2 * row, 3: row, 5: col} if col else {2: 2 * logn, 5: logn}
+        bfly = sum(stages.values()) * half * polys * limbs
+        t_peak = sum(c * half * polys * limbs / (peaks[s]["peak"] * 1e9) for s, c in stages.items())
+        ms = res["fused_ms"]
+        ach = bfly / (ms * 1e-3) / 1e9
+        passes_bytes = 80 * n * polys * limbs  # COL a, COL b, fused ROW (a, b, addend in; one out), inverse COL
+        min_bytes = 32 * n * polys * limbs     # a, b, addend read once, result written once
+        res["roofline"] = {
+            "valu": {"achieved": ach, "peak": bfly / t_peak / 1e9, "unit": "Gbfly/s",
+                     "frac": t_peak / (ms * 1e-3),
+                     "peak_source": "schedule peaks weighted by the product's stages: "
+                                    + ", ".join(f"{ax.SCHEDULE_NAMES[s]} x {c}" for s, c in stages.items())},
+            "hbm": {"achieved": passes_bytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": passes_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "bytes_per_product": 80 * n, "minimum_bytes_per_product": 32 * n,
+                    "frac_at_minimum_bytes": min_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
     return res
 
 
-def uneven_leg(ax, torch, params, jobs=512, reps=20, warm=10, seed=0):
+def uneven_leg(ax, torch, params, jobs=512, reps=20, warm=10, seed=0, peaks=None):
     """512 jobs in one graph replay: N = 2^(12+U{0..5}), limbs U{1..40},
     moduli of 30..60 bits, forward and inverse mixed."""
     rng = np.random.default_rng(seed)
@@ -186,15 +221,19 @@ def uneven_leg(ax, torch, params, jobs=512, reps=20, warm=10, seed=0):
         inverse = bool(rng.integers(0, 2))
         spec.append((logn, limbs, inverse, int(rng.integers(0, 40))))
         total_words += limbs << logn
-    src = torch.randint(0, 1 << 29, (total_words,), device="cuda", dtype=torch.int64)
+    # residues uniform in [0, q) for each limb's own modulus
+    src = torch.empty((total_words,), device="cuda", dtype=torch.int64)
     dst = torch.empty_like(src)
+    g = torch.Generator(device="cuda").manual_seed(seed)
     blist = []
     off = 0
     limbs_total = 0
     for logn, limbs, inverse, k in spec:
         n = 1 << logn
         for l in range(limbs):
-            blist.append((pools[logn][(k + l) % 40], dst[off:off + n], src[off:off + n], inverse, 1))
+            plan = pools[logn][(k + l) % 40]
+            src[off:off + n] = torch.randint(0, plan.modulus, (n,), generator=g, device="cuda", dtype=torch.int64)
+            blist.append((plan, dst[off:off + n], src[off:off + n], inverse, 1))
             off += n
         limbs_total += limbs
     batch = ax.Batch(blist)
@@ -212,10 +251,14 @@ def uneven_leg(ax, torch, params, jobs=512, reps=20, warm=10, seed=0):
         batch.launch_eager()
     torch.cuda.synchronize()
     ems = (time.perf_counter() - t0) * 1e3 / reps
+    infos = [batch.kernel_info(k) for k in range(batch.kernel_count())]
     batch.close()
-    return {"jobs": jobs, "limb_transforms": limbs_total, "words": total_words, "graph_ms": ms,
-            "eager_ms": ems, "limbs_per_s": limbs_total / (ms * 1e-3),
-            "coeff_per_s": total_words / (ms * 1e-3)}
+    out = {"jobs": jobs, "limb_transforms": limbs_total, "words": total_words, "graph_ms": ms,
+           "eager_ms": ems, "limbs_per_s": limbs_total / (ms * 1e-3),
+           "coeff_per_s": total_words / (ms * 1e-3), "inputs": "uniform in [0, q) per limb"}
+    if peaks:
+        out["roofline"] = mix_roofline(infos, ms, peaks)
+    return out
 
 
 def serve_leg(ax, seconds):
@@ -240,21 +283,136 @@ def p99_summary(serving):
              "p50_us": x["p50_us"], "p99_us": x["p99_us"], "mix": serving.get("mix")} for x in serving["loads"]]
 
 
+def sync_time(torch, fn, steps):
+    """ms per call of fn over `steps` calls, bracketed by device synchronizations."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def steady_warmup(torch, fn, min_steps, block=5, tol=0.02, max_seconds=3.0):
+    """Runs fn at least min_steps times, then in blocks of `block` calls until
+    two consecutive blocks take the same time within `tol` (the clock has
+    settled), at most max_seconds.  A cold GPU runs its first dispatches 2-4x
+    slower while clocks ramp, so a fixed short warm-up leaves the timed steps
+    on the ramp."""
+    for _ in range(min_steps):
+        fn()
+    done = min_steps
+    t_end = time.perf_counter() + max_seconds
+    prev, last = None, None
+    converged = False
+    while time.perf_counter() < t_end:
+        last = sync_time(torch, fn, block)
+        done += block
+        if prev is not None and abs(last - prev) <= tol * prev:
+            converged = True
+            break
+        prev = last
+    return {"requested": min_steps, "done": done, "converged": converged, "rule":
+            f"at least the requested steps, then blocks of {block} until two consecutive blocks agree "
+            f"within {tol:.0%} (at most {max_seconds:.0f} s)", "last_block_ms_per_step": last}
+
+
+def kernel_durations(torch, batch, reps, stream):
+    """Average duration of each launch of a replay: reps launches of the
+    kernel back to back between one pair of HIP events on the stream it runs
+    on (the host's launch latency hides behind the running kernels)."""
+    out = []
+    with torch.cuda.stream(stream):
+        for k in range(batch.kernel_count()):
+            for _ in range(3):
+                batch.launch_kernel(k, stream)  # warm
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(reps):
+                batch.launch_kernel(k, stream)
+            b.record(stream)
+            stream.synchronize()
+            out.append(a.elapsed_time(b) / reps)
+    return out
+
+
+def schedule_peaks(ax, seconds=0.3):
+    """Register-resident butterfly rate of every pass schedule: the
+    microbenchmark back to back for ~`seconds`, median of the second half of
+    the runs (the first runs after a host-side gap start at a lower clock and
+    read ~10 % low; the first run is kept beside the peak)."""
+    peaks = {}
+    for s, name in enumerate(ax.SCHEDULE_NAMES):
+        runs, secs, t_end = [], [], time.perf_counter() + seconds
+        while time.perf_counter() < t_end or len(runs) < 8:
+            r, sec = ax.butterfly_peak(s)
+            runs.append(r / 1e9)
+            secs.append(sec)
+        peaks[s] = {"schedule": name, "peak": float(np.median(runs[len(runs) // 2:])),
+                    "seconds": float(np.median(secs[len(secs) // 2:])), "first_run": runs[0], "runs": len(runs)}
+    return peaks
+
+
+def kernel_roofline(info, ms, peaks):
+    """One pass launch against its bounds: butterflies/s over the
+    register-resident peak of the same schedule, and algorithmic bytes/s over
+    HBM peak.  The bound is the larger of the two fractions."""
+    pk = peaks[info["schedule"]]
+    bfly = info["butterflies"] / (ms * 1e-3) / 1e9
+    gbs = info["bytes"] / (ms * 1e-3) / 1e9
+    valu = {"achieved": bfly, "peak": pk["peak"], "unit": "Gbfly/s", "frac": bfly / pk["peak"],
+            "peak_source": f"axhel_hip_ubench_schedule_peak({info['schedule']}) = {pk['schedule']}: "
+                           f"median of the last {pk['runs'] - pk['runs'] // 2} of {pk['runs']} back-to-back runs",
+            "peak_first_run": pk["first_run"]}
+    hbm = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+    bound = "valu" if valu["frac"] >= hbm["frac"] else "hbm"
+    return {"ms": ms, "info": info, "valu": valu, "hbm": hbm, "bound": bound,
+            "frac": max(valu["frac"], hbm["frac"])}
+
+
+def mix_roofline(infos, ms, peaks):
+    """A whole replay of several launches (uneven degrees, both directions):
+    butterflies/s against the time the same launches would take at their
+    schedules' peaks, and algorithmic pass bytes/s against HBM peak."""
+    bfly = sum(i["butterflies"] for i in infos)
+    t_peak = sum(i["butterflies"] / (peaks[i["schedule"]]["peak"] * 1e9) for i in infos)
+    byts = sum(i["bytes"] for i in infos)
+    ach = bfly / (ms * 1e-3) / 1e9
+    gbs = byts / (ms * 1e-3) / 1e9
+    return {"valu": {"achieved": ach, "peak": bfly / t_peak / 1e9, "unit": "Gbfly/s", "frac": t_peak / (ms * 1e-3),
+                     "peak_source": "schedule peaks weighted by each launch's butterflies"},
+            "hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                    "bytes": byts, "note": "16 N bytes per limb per pass"},
+            "launches": len(infos)}
+
+
+def batch_leg(ax, torch, batch, limbs, steps, warmup, stream, peaks, reps):
+    """A batch timed as graph replays on `stream` after a steady warm-up,
+    with each launch's roofline."""
+    wu = steady_warmup(torch, lambda: batch.launch(stream), warmup)
+    ms = sync_time(torch, lambda: batch.launch(stream), steps)
+    kd = kernel_durations(torch, batch, reps, stream)
+    kern = [kernel_roofline(batch.kernel_info(k), kd[k], peaks) for k in range(len(kd))]
+    return {"ms_per_step": ms, "limbs_per_s": limbs / (ms * 1e-3), "warmup": wu, "kernels": kern}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--polys", type=int, default=64)
     ap.add_argument("--limbs", type=int, default=24)
     ap.add_argument("--logn", type=int, default=16)
     ap.add_argument("--plain-limbs", type=int, default=384, help="limbs timed on the plain per-call path")
+    ap.add_argument("--kernel-reps", type=int, default=20, help="back-to-back launches per kernel duration")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eltwise", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-polymul", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-lazy", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run one 2^27-word EltwiseReduceMod(4->2) (1 GiB in, 1 GiB out): "
@@ -291,107 +449,58 @@ def main():
     for l, q in enumerate(primes):
         src[:, l, :] = torch.randint(0, q, (P, n), generator=g, device="cuda", dtype=torch.int64)
     dst = torch.empty_like(src)
+    aux = torch.empty_like(src)
     torch.cuda.synchronize()
 
     jobs = [(plans[l], dst[p, l], src[p, l], False, 1) for p in range(P) for l in range(L)]
     batch = ax.Batch(jobs)
     limbs = P * L
-
-    # Timed steps: graph replays on the default stream, as Batch.launch()
-    # issues them.  Beside it: the same replays on a separate stream, and the
-    # same kernels as plain launches (no graph) on that stream.
     run_stream = torch.cuda.Stream()
-    for _ in range(args.warmup):
-        batch.launch()
-        batch.launch(run_stream)
-        batch.launch_eager(run_stream)
+    launch = lambda: batch.launch()
+
+    # Warm-up on the headline step until its time settles, then the
+    # variants (graph on a separate stream, eager launches, one RNS-context
+    # call), then the headline: graph replays on the default stream, as
+    # Batch.launch() issues them, timed last so no variant rides the ramp.
+    warm = steady_warmup(torch, launch, args.warmup)
+    log(f"warm-up: {warm['done']} steps (converged {warm['converged']}, "
+        f"last block {warm['last_block_ms_per_step']:.3f} ms/step)")
+    stream_ms = sync_time(torch, lambda: batch.launch(run_stream), args.steps)
+    log(f"graph replay on a separate stream: {stream_ms:.3f} ms/step")
+    eager_ms = sync_time(torch, lambda: batch.launch_eager(run_stream), args.steps)
+    log(f"same step as eager launches: {eager_ms:.3f} ms/step")
+    # the same step as one RNS-context call (job tables built on the device
+    # per call, no graph): what a SEAL-style limb loop gets without a Batch
+    rns = ax.RNS(n, primes)
+    rns_call = lambda: rns.ntt(dst, src, P, False, 1, stream=run_stream)
+    for _ in range(3):
+        rns_call()
+    rns_ms = sync_time(torch, rns_call, args.steps)
+    log(f"same step as one RNS-context call: {rns_ms:.3f} ms/step")
+    rns.close()
+    for _ in range(5):
+        launch()
+    # headline: exactly `steps` replays bracketed by device synchronizations
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        batch.launch()
+        launch()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms = el * 1e3 / args.steps
     value = limbs * args.steps / el
     log(f"batched forward (graph replay, default stream): {ms:.3f} ms/step, {value:.4g} limbs/s")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.launch(run_stream)
-    torch.cuda.synchronize()
-    stream_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    log(f"graph replay on a separate stream: {stream_ms:.3f} ms/step")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.launch_eager(run_stream)
-    torch.cuda.synchronize()
-    eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    log(f"same step as eager launches: {eager_ms:.3f} ms/step")
 
-    # the same step as one RNS-context call (job tables built on the device
-    # per call, no graph): what a SEAL-style limb loop gets without a Batch
-    rns = ax.RNS(n, primes)
-    for _ in range(10):
-        rns.ntt(dst, src, P, False, 1, stream=run_stream)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rns.ntt(dst, src, P, False, 1, stream=run_stream)
-    torch.cuda.synchronize()
-    rns_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    log(f"same step as one RNS-context call: {rns_ms:.3f} ms/step")
-    rns.close()
-
-    # inverse (reported beside)
-    ibatch = ax.Batch([(plans[l], src[p, l], dst[p, l], True, 1) for p in range(P) for l in range(L)])
-    ibatch.launch(run_stream)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ibatch.launch(run_stream)
-    torch.cuda.synchronize()
-    inv_rate = limbs * args.steps / (time.perf_counter() - t0)
-    # restore the forward inputs (inverse of forward is the identity)
-    batch.launch()
-    ibatch.launch()
-    torch.cuda.synchronize()
-
-    # per-kernel durations with HIP events on the stream the kernels run on
+    # per-kernel durations with HIP events on the stream the kernels run on,
+    # and the register-resident peak of each pass schedule
     stream = torch.cuda.Stream()
-    kcount = batch.kernel_count()
-    reps = max(5, min(20, args.steps))
-    kdur = []
-    with torch.cuda.stream(stream):
-        for k in range(kcount):
-            # reps launches of kernel k back to back between one pair of
-            # events on its stream (sustained duration of one launch)
-            batch.launch_kernel(k, stream)  # warm
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            for _ in range(reps):
-                batch.launch_kernel(k, stream)
-            b.record(stream)
-            stream.synchronize()
-            kdur.append(a.elapsed_time(b) / reps)
+    kdur = kernel_durations(torch, batch, args.kernel_reps, stream)
+    peaks = schedule_peaks(ax)
     log("kernel avg ms:", [round(x, 4) for x in kdur])
+    log("schedule peaks Gbfly/s:", {v["schedule"]: round(v["peak"], 1) for v in peaks.values()})
+    kern = [kernel_roofline(batch.kernel_info(k), kdur[k], peaks) for k in range(len(kdur))]
     dom = int(np.argmax(kdur))
-    # Algorithmic work of one pass launch: each of the two passes runs half of
-    # the log2(N) stages, (N/2)*(log2(N)/2) butterflies per limb, and reads and
-    # writes every coefficient once (16 N bytes per limb).
-    bfly = (n // 2) * (args.logn / 2.0) * limbs
-    alg_bytes = 16.0 * n * limbs
-    achieved_bfly = bfly / (kdur[dom] * 1e-3) / 1e9
-    achieved_gbs = alg_bytes / (kdur[dom] * 1e-3) / 1e9
-    # register-resident butterfly peak: the microbenchmark back to back for
-    # ~0.3 s, median of the second half.  The first runs after the host-side
-    # gap start at a lower clock and read ~10 % low; the first run is
-    # reported beside the peak.
-    runs, secs, t_end = [], [], time.perf_counter() + 0.3
-    while time.perf_counter() < t_end or len(runs) < 8:
-        r, sec = ax.butterfly_peak()
-        runs.append(r / 1e9)
-        secs.append(sec)
-    peak_bfly = float(np.median(runs[len(runs) // 2:]))
-    peak_secs = float(np.median(secs[len(secs) // 2:]))
+    kd = kern[dom]
     cfg = ax.config_get(n, ax.config_selected(n))
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "ntt_n65536_64x24_hbm_bytes.txt")
@@ -400,38 +509,42 @@ def main():
             if line.startswith("traffic_bytes"):
                 kv = dict(x.split("=") for x in line.split()[1:])
                 traffic = float(kv.get(f"pass{dom}", "nan"))
+    log(f"dominant pass {dom} ({kd['info']['schedule_name']}): {kd['valu']['achieved']:.1f} of "
+        f"{kd['valu']['peak']:.1f} Gbfly/s = {kd['valu']['frac']:.3f}; {kd['hbm']['achieved']:.0f} GB/s = "
+        f"{kd['hbm']['frac']:.3f} of HBM; bound {kd['bound']}")
+
+    # lazy transforms (SEAL's hot calls: ComputeForward(.., 4, 4) and
+    # ComputeInverse(.., 2, 2)) and the normalized inverse, as batches
+    legs = {}
+    leg_specs = [("inverse", aux, dst, True, 1)]
+    if not args.no_lazy:
+        leg_specs += [("lazy_forward", aux, src, False, 4), ("lazy_inverse", aux, dst, True, 2)]
+    for name, out_t, in_t, inverse, factor in leg_specs:
+        b = ax.Batch([(plans[l], out_t[p, l], in_t[p, l], inverse, factor) for p in range(P) for l in range(L)])
+        legs[name] = batch_leg(ax, torch, b, limbs, args.steps, args.warmup, run_stream, peaks, args.kernel_reps)
+        b.close()
+        lk = legs[name]["kernels"]
+        ld = int(np.argmax([k["ms"] for k in lk]))
+        legs[name]["dominant"] = ld
+        log(f"{name}: {legs[name]['ms_per_step']:.3f} ms/step, {legs[name]['limbs_per_s']:.4g} limbs/s; "
+            f"dominant pass {ld} ({lk[ld]['info']['schedule_name']}) valu {lk[ld]['valu']['frac']:.3f} "
+            f"hbm {lk[ld]['hbm']['frac']:.3f}")
+
     # VALU issue: wave64 VALU instructions per launch (SQ_INSTS_VALU of the
-    # committed counter profile of this workload) over the launch duration,
-    # against the rate the butterfly microbenchmark sustains (its own
-    # SQ_INSTS_VALU per launch from the same profile over its warmed launch
-    # duration); the nominal 4-cycle figure is reported beside it.
+    # committed counter profile of this workload) over the launch duration
     valu_issue = None
     vpath = os.path.join(ROOT, "profiles", "ntt_n65536_64x24_valu_counters.txt")
     if os.path.exists(vpath) and args.logn == 16 and P == 64 and L == 24:
-        insts = ub_insts = None
+        insts = None
         for line in open(vpath):
-            f = line.split()
             if line.startswith("valu_insts"):
-                kv = dict(x.split("=") for x in f[1:])
+                kv = dict(x.split("=") for x in line.split()[1:])
                 insts = float(kv.get(f"pass{dom}", "nan"))
-            elif len(f) == 4 and f[0] == "bfly_peak_kernel" and f[2] == "SQ_INSTS_VALU":
-                ub_insts = float(f[3])
-        if insts is not None and ub_insts is not None:
+        if insts is not None:
             ach = insts / (kdur[dom] * 1e-3) / 1e9
-            # each microbenchmark call makes a 64-iteration warm-up launch and
-            # the timed 2048-iteration launch of the same grid; the profile
-            # averages the two, so the timed launch holds 2 / (1 + 64/2048)
-            # of that average
-            ub_timed = ub_insts * 2.0 / (1.0 + 64.0 / 2048.0)
-            ub_rate = ub_timed / peak_secs / 1e9
-            valu_issue = {"achieved": ach, "peak": ub_rate, "unit": "G wave64 VALU instr/s", "frac": ach / ub_rate,
-                          "peak_source": "butterfly microbenchmark's SQ_INSTS_VALU per timed launch over its warmed launch time",
-                          "ubench_valu_per_butterfly": ub_timed * 64.0 / (peak_bfly * 1e9 * peak_secs),
-                          "nominal": VALU_ISSUE_NOMINAL, "frac_of_nominal": ach / VALU_ISSUE_NOMINAL,
-                          "insts_per_launch": insts}
-    log(f"dominant pass {dom}: {achieved_bfly:.1f} Gbfly/s of {peak_bfly:.1f} peak "
-        f"(first of {len(runs)} runs {runs[0]:.1f}); {achieved_gbs:.0f} GB/s"
-        + (f"; VALU issue {valu_issue['frac']:.2f} of peak" if valu_issue else ""))
+            valu_issue = {"achieved": ach, "unit": "G wave64 VALU instr/s", "nominal": VALU_ISSUE_NOMINAL,
+                          "frac_of_nominal": ach / VALU_ISSUE_NOMINAL, "insts_per_launch": insts,
+                          "valu_per_butterfly": insts * 64.0 / kd["info"]["butterflies"]}
 
     # plain per-call path: one synchronous ComputeForward per limb
     pl = min(args.plain_limbs, limbs)
@@ -462,14 +575,14 @@ def main():
         elt = eltwise_leg(ax, torch, src, dst, primes[0], reps=max(5, min(20, args.steps)))
         log("eltwise GB/s:", {k: round(v["GB/s"], 1) for k, v in elt.items()})
 
-    del batch, ibatch
+    del batch, aux
     pmul = None
     if not args.no_polymul:
-        pmul = polymul_leg(ax, torch, params)
+        pmul = polymul_leg(ax, torch, params, peaks=peaks)
         log("polymul:", json.dumps(pmul))
     uneven = None
     if not args.no_uneven:
-        uneven = uneven_leg(ax, torch, params)
+        uneven = uneven_leg(ax, torch, params, peaks=peaks)
         log("uneven batch:", json.dumps(uneven))
 
     serving = None
@@ -482,6 +595,7 @@ def main():
         cpu = cpu_baseline(n, primes, psis, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
 
+    bound_obj = kd["valu"] if kd["bound"] == "valu" else kd["hbm"]
     out = {
         "metric": "NTT limbs/sec at N=2^16,L=24 (batched vs plain); p99 latency at fixed load",
         "value": value,
@@ -489,6 +603,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_done": warm,
         "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
@@ -501,20 +616,24 @@ def main():
         "plain_limbs_per_s": plain,
         "plain_host_buffers_limbs_per_s": plain_host,
         "batched_vs_plain": value / plain,
-        "inverse_limbs_per_s": inv_rate,
+        "inverse_limbs_per_s": legs["inverse"]["limbs_per_s"],
+        "lazy_forward_limbs_per_s": legs.get("lazy_forward", {}).get("limbs_per_s"),
+        "lazy_inverse_limbs_per_s": legs.get("lazy_inverse", {}).get("limbs_per_s"),
         "kernel_ms": kdur,
+        "kernels": kern,
+        "legs": legs,
+        "schedule_peaks": peaks,
         "eager_ms_per_step": eager_ms,
         "rns_context_ms_per_step": rns_ms,
         "separate_stream_graph_ms_per_step": stream_ms,
-        "roofline": {"bound": "valu", "achieved": achieved_bfly, "peak": peak_bfly, "unit": "Gbfly/s",
-                     "frac": achieved_bfly / peak_bfly, "traffic": traffic,
-                     "kernel": f"ntt pass {dom} of {kcount} ({kdur[dom]:.4f} ms)",
-                     "peak_source": "register-resident Harvey butterflies, all CUs (axhel_hip_ubench_butterfly_peak), "
-                                    f"median of the last {len(runs) - len(runs) // 2} of {len(runs)} back-to-back runs",
-                     "peak_first_run": runs[0],
-                     "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": achieved_gbs / HBM_PEAK_GBS},
-                     "valu_issue": valu_issue},
+        "roofline": {"bound": kd["bound"], "achieved": bound_obj["achieved"], "peak": bound_obj["peak"],
+                     "unit": bound_obj["unit"], "frac": kd["frac"], "traffic": traffic,
+                     "kernel": f"ntt pass {dom} of {len(kdur)} ({kdur[dom]:.4f} ms, "
+                               f"{kd['info']['schedule_name']}, stacked={kd['info']['stacked']})",
+                     "rule": "larger of the VALU fraction (butterflies/s over the register-resident peak of the "
+                             "pass's own butterfly schedule) and the HBM fraction (16 N bytes per limb per pass "
+                             "over 8 TB/s)",
+                     "valu": kd["valu"], "hbm": kd["hbm"], "valu_issue": valu_issue},
         "eltwise": elt,
         "polymul": pmul,
         "uneven_batch": uneven,

@@ -158,6 +158,29 @@ axhel_hip_status axhel_hip_ntt_batch_launch_kernel(axhel_hip_batch *batch, int i
    jobs to whole stacks leaves at most one slot in eight empty.  Diagnostic;
    no reference counterpart. */
 int axhel_hip_ntt_batch_stacked_groups(const axhel_hip_batch *batch);
+/* What the index-th launch of a replay computes (per-kernel rooflines).
+   limbs counts the launch's limb-transforms (empty stack slots excluded);
+   butterflies = (degree / 2) * log_bits * limbs; bytes = 16 * degree * limbs,
+   every coefficient read once and written once by the pass.  schedule is
+   the AXHEL_HIP_UBENCH_* butterfly schedule of the pass's stages
+   (include/axhel_hip_ubench.h).  Diagnostic; no reference counterpart. */
+typedef struct axhel_hip_ntt_kernel_info {
+    uint64_t degree;
+    uint32_t limbs;
+    uint32_t pass;     /* 0 = the transform's first pass */
+    uint32_t passes;   /* passes of the transform */
+    uint32_t log_bits; /* index bits (butterfly stages) this pass covers */
+    int32_t inverse;
+    int32_t normalize; /* the pass folds the output normalization in */
+    int32_t relaxed;   /* relaxed forward schedule */
+    int32_t stacked;   /* ROW pass over stacks of same-plan jobs */
+    int32_t schedule;
+    int32_t reserved;
+    uint64_t butterflies;
+    uint64_t bytes;
+} axhel_hip_ntt_kernel_info;
+axhel_hip_status axhel_hip_ntt_batch_kernel_info(const axhel_hip_batch *batch, int index,
+                                                 axhel_hip_ntt_kernel_info *info);
 /* One-shot batch: create, launch eagerly on `stream`, synchronize, destroy. */
 axhel_hip_status axhel_hip_ntt_batch_run(const axhel_hip_ntt_job *jobs, size_t count, void *stream);
 
@@ -227,12 +250,15 @@ typedef enum axhel_hip_backend {
 } axhel_hip_backend;
 
 typedef struct axhel_hip_serve_config {
-    uint64_t memory_budget_bytes; /* device bytes for staging host operands; 0 = half the free HBM */
+    uint64_t memory_budget_bytes; /* device bytes for staging host operands and the server's descriptor
+                                     buffers; 0 = half the free HBM */
     uint32_t max_batch_limbs;     /* limbs packed into one device batch; 0 = 8192 */
     uint32_t slots;               /* device batches in flight; 0 = 4 */
     uint32_t backend;             /* axhel_hip_backend */
     uint32_t null_latency_us;     /* NULL backend: simulated batch duration */
     uint64_t fail_every_alloc;    /* fault injection: every k-th staging allocation fails; 0 = never */
+    uint64_t fail_every_batch;    /* fault injection: every k-th device batch fails after its staging copies
+                                     are queued (its requests end AXHEL_HIP_NO_MEMORY); 0 = never */
 } axhel_hip_serve_config;
 
 /* One request: `limbs` vectors of `degree` words, limb l at offset l*degree
@@ -257,9 +283,11 @@ typedef struct axhel_hip_request {
 typedef struct axhel_hip_serve_stats {
     uint64_t submitted, completed, rejected, timed_out, cancelled, failed;
     uint64_t limbs_completed, batches;
-    uint64_t bytes_in_use, peak_bytes_in_use;
+    uint64_t bytes_in_use;      /* staging of admitted requests */
+    uint64_t peak_bytes_in_use; /* peak of staging plus descriptor buffers */
     double p50_us, p99_us, mean_us, max_us; /* arrival -> completion, completed requests */
     double first_arrival_to_last_completion_s;
+    uint64_t descriptor_bytes; /* the slots' device descriptor buffers (charged to the budget) */
 } axhel_hip_serve_stats;
 
 typedef struct axhel_hip_server axhel_hip_server;

@@ -99,7 +99,8 @@ def test_uneven_512_jobs_full_size_equal_plain_calls():
         lazy = bool(rng.integers(0, 2))
         spec.append((logn, limbs, inverse, lazy, int(rng.integers(0, 40))))
         total += limbs << logn
-    src = torch.randint(0, 1 << 29, (total,), device="cuda", dtype=torch.int64)  # below every modulus
+    src = torch.empty((total,), device="cuda", dtype=torch.int64)  # residues uniform in [0, q) per limb
+    g = torch.Generator(device="cuda").manual_seed(12)
     out = torch.full_like(src, -1)
     ref = torch.full_like(src, -1)
     jobs, placed = [], []
@@ -109,6 +110,7 @@ def test_uneven_512_jobs_full_size_equal_plain_calls():
         of = (2 if inverse else 4) if lazy else 1
         for l in range(limbs):
             plan, q = pools[logn][(k + l) % 40]
+            src[off:off + n] = torch.randint(0, q, (n,), generator=g, device="cuda", dtype=torch.int64)
             jobs.append((plan, out[off:off + n], src[off:off + n], inverse, of))
             placed.append((plan, q, off, n, inverse, of))
             off += n

@@ -189,3 +189,35 @@ def test_random_mixed_polymul_jobs():
         A, B = O.compute_forward(a, t, 1, 1), O.compute_forward(b, t, 1, 1)
         D = O.fma_mod(O.mul_mod(A, B, q, 1), s, c if has_c else None, q, 1)
         assert np.array_equal(host(views[3]), O.compute_inverse(D, t, 1, 1)), (i, n, q.bit_length(), has_c)
+
+
+@pytest.mark.parametrize("create_cfg,launch_cfg", [(1, 2), (2, 1)])
+def test_polymul_keeps_the_configuration_it_was_created_with(create_cfg, launch_cfg):
+    """At 2^13 configurations 0-1 run one pass and 2-3 run two, and the fused
+    product's descriptors depend on the pass count.  A product created under
+    one and launched after the degree's selection changed to the other still
+    equals the five-call sequence."""
+    torch = torch_mod()
+    n, limbs = 1 << 13, 3
+    qs = params.rns_primes(n, limbs)
+    psis = [params.find_primitive_2n_root(n, q) for q in qs]
+    plans = [ax.NTT(n, q, p) for q, p in zip(qs, psis)]
+    rng = np.random.default_rng(41)
+    A, B, C = (np.stack([params.uniform(rng, n, q) for q in qs]) for _ in range(3))
+    S = [int(params.uniform(rng, 1, q)[0]) for q in qs]
+    da, db, dc = (dev(X.ravel()).view(limbs, n) for X in (A, B, C))
+    out = empty_dev(limbs * n).view(limbs, n)
+    scr = empty_dev(limbs * n).view(limbs, n)
+    jobs = [(plans[l], out[l], da[l], db[l], dc[l], S[l], scr[l]) for l in range(limbs)]
+    try:
+        ax.config_select(n, create_cfg)
+        pm = ax.Polymul(jobs, fused=True)
+        ax.config_select(n, launch_cfg)
+        pm.launch()
+        got = host(out).reshape(limbs, n)
+        pm.close()
+    finally:
+        ax.config_select(n, -1)
+    for l in range(limbs):
+        want = oracle_polymul(A[l], B[l], C[l], S[l], O.Tables(n, qs[l], psis[l]), qs[l])
+        assert np.array_equal(got[l], want), (create_cfg, launch_cfg, l)

@@ -184,3 +184,118 @@ def test_requests_on_8_byte_aligned_buffers():
         assert np.array_equal(host(dst), R["want"]), (i, R["kind"], R["n"], inplace)
         assert bool((base[0] == 0).item())
     server.close()
+
+
+def test_requests_after_a_failed_batch_are_correct():
+    """Every second device batch fails after its host operands' staging
+    copies were queued (injected).  The server drains that stream before its
+    staging blocks go back to the pool, so the next request, which reuses
+    them, still equals the oracle; the failed requests end NO_MEMORY."""
+    rng = np.random.default_rng(17)
+    server = ax.Server(slots=1, memory_budget_bytes=256 << 20, fail_every_batch=2)
+    n, limbs = 1 << 15, 4
+    qs = params.rns_primes(n, limbs)
+    psis = [params.find_primitive_2n_root(n, q) for q in qs]
+    plans = [server.plan(n, q, p) for q, p in zip(qs, psis)]
+    tabs = [O.Tables(n, q, p) for q, p in zip(qs, psis)]
+    got = []
+    for i in range(6):  # one request per batch: submit, then wait
+        a = np.concatenate([params.uniform(rng, n, q) for q in qs])
+        r = np.zeros_like(a)
+        st, t = server.submit(ax.OP_NTT_FORWARD, n, limbs, r, a, plans=plans, output_mod_factor=1)
+        assert st == ax.OK
+        res = server.wait(t)
+        got.append(res)
+        if res == ax.OK:
+            for l in range(limbs):
+                assert np.array_equal(r[l * n:(l + 1) * n], O.compute_forward(a[l * n:(l + 1) * n], tabs[l])), (i, l)
+    assert got == [ax.OK, ax.NO_MEMORY] * 3, got
+    stt = server.stats()
+    assert stt["failed"] == 3 and stt["bytes_in_use"] == 0 and stt["descriptor_bytes"] > 0
+    server.close()
+
+
+def test_wide_requests_alone_equal_mix_equal_oracle():
+    """The serving mix at its configured width: 1..48 limbs per request and
+    N in 2^13..2^17, so single requests (up to 6.3 M words) and whole batches
+    cross the 2^21-word switch to the throughput configuration.  Each
+    result is the same alone and in the concurrent mix, and its first and
+    last limb equal the oracle."""
+    torch = torch_mod()
+    rng = np.random.default_rng(2026)
+    server = ax.Server(slots=4)
+    kinds = ["fwd", "inv", "mul", "fwd", "inv", "fma"]
+    reqs, cache = [], {}
+    for i in range(18):
+        n = 1 << int(rng.integers(13, 18))
+        limbs = 48 if i < 2 else int(rng.integers(1, 49))
+        kind = kinds[i % len(kinds)]
+        qs = params.rns_primes(n, limbs)
+        psis = [params.find_primitive_2n_root(n, q) for q in qs]
+        bound = {"fwd": 4, "inv": 2}.get(kind, 1)
+        a = np.concatenate([params.uniform(rng, n, bound * q) for q in qs])
+        b = np.concatenate([params.uniform(rng, n, q) for q in qs])
+        sc = [int(params.uniform(rng, 1, q)[0]) for q in qs]
+        lazy = bool(i % 4 == 3)
+        want = {}
+        for l in sorted({0, limbs - 1}):
+            q, psi = qs[l], psis[l]
+            if kind in ("fwd", "inv") and (n, q) not in cache:
+                cache[(n, q)] = O.Tables(n, q, psi)
+            al, bl = a[l * n:(l + 1) * n], b[l * n:(l + 1) * n]
+            if kind == "fwd":
+                want[l] = O.compute_forward(al, cache[(n, q)], 4, 4 if lazy else 1)
+            elif kind == "inv":
+                want[l] = O.compute_inverse(al, cache[(n, q)], 2, 2 if lazy else 1)
+            elif kind == "mul":
+                want[l] = O.mul_mod(al, bl, q, 1)
+            else:
+                want[l] = O.fma_mod(al, sc[l], bl, q, 1)
+        plans = [server.plan(n, q, psi) for q, psi in zip(qs, psis)] if kind in ("fwd", "inv") else None
+        reqs.append(dict(kind=kind, n=n, limbs=limbs, qs=qs, plans=plans, sc=sc, lazy=lazy, want=want,
+                         da=dev(a), db=dev(b)))
+    assert max(R["n"] * R["limbs"] for R in reqs) > 1 << 21
+
+    def run(R):
+        out = empty_dev(R["n"] * R["limbs"])
+        common = dict(degree=R["n"], limbs=R["limbs"], result=out, operand1=R["da"])
+        if R["kind"] == "fwd":
+            st, t = server.submit(ax.OP_NTT_FORWARD, plans=R["plans"], output_mod_factor=4 if R["lazy"] else 1,
+                                  **common)
+        elif R["kind"] == "inv":
+            st, t = server.submit(ax.OP_NTT_INVERSE, plans=R["plans"], output_mod_factor=2 if R["lazy"] else 1,
+                                  **common)
+        elif R["kind"] == "mul":
+            st, t = server.submit(ax.OP_MUL, operand2=R["db"], moduli=R["qs"], input_mod_factor=1, **common)
+        else:
+            st, t = server.submit(ax.OP_FMA, operand2=R["db"], moduli=R["qs"], scalars=R["sc"], input_mod_factor=1,
+                                  **common)
+        assert st == ax.OK, (R["kind"], st)
+        return out, t
+
+    alone = []
+    for R in reqs:
+        out, t = run(R)
+        assert server.wait(t) == ax.OK
+        alone.append(host(out))
+    outs, tickets = [None] * len(reqs), [None] * len(reqs)
+
+    def client(idx):
+        for i in idx:
+            outs[i], tickets[i] = run(reqs[i])
+
+    ths = [threading.Thread(target=client, args=(list(range(k, len(reqs), 3)),)) for k in range(3)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    for t in tickets:
+        assert server.wait(t) == ax.OK
+    for i, R in enumerate(reqs):
+        mix = host(outs[i])
+        assert np.array_equal(alone[i], mix), (i, R["kind"], R["n"], R["limbs"])
+        n = R["n"]
+        for l, w in R["want"].items():
+            assert np.array_equal(mix[l * n:(l + 1) * n], w), (i, R["kind"], n, R["limbs"], l)
+    assert server.stats()["failed"] == 0
+    server.close()

@@ -137,6 +137,24 @@ def test_staging_allocation_failures_are_reported():
     s.close()
 
 
+def test_batch_failures_are_reported_and_release_memory():
+    """Every second batch fails after its staging was reserved: its requests
+    end NO_MEMORY, the others OK, and all staging returns to the pool."""
+    s = ax.Server(backend=ax.BACKEND_NULL, null_latency_us=200, slots=1, memory_budget_bytes=256 * MIB,
+                  fail_every_batch=2)
+    keep, res = [], []
+    for _ in range(8):  # one request per batch: submit, then wait
+        st, t, bufs = req(s)
+        assert st == ax.OK
+        keep.append(bufs)
+        res.append(s.wait(t))
+    assert res == [ax.OK, ax.NO_MEMORY] * 4, res
+    stt = s.stats()
+    assert stt["failed"] == 4 and stt["completed"] == 4
+    assert stt["bytes_in_use"] == 0
+    s.close()
+
+
 def test_invalid_requests_are_refused():
     s = ax.Server(backend=ax.BACKEND_NULL)
     a = np.zeros(N, dtype=np.uint64)
