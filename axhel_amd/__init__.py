@@ -162,6 +162,7 @@ SIGNATURES = {
     "axhel_hip_rns_destroy": (_st, [_p]),
     "axhel_hip_rns_plan": (_p, [_p, ctypes.c_uint32]),
     "axhel_hip_rns_ntt": (_st, [_p, _p, _p, _u64, ctypes.c_int, _u64, _p]),
+    "axhel_hip_rns_ntt_noalloc": (_st, [_p, _p, _p, _u64, ctypes.c_int, _u64, _p]),
     "axhel_hip_rns_eltwise": (_st, [_p, ctypes.c_int, _p, _p, _p, _p, _u64, _u64, _u64, _p]),
     "axhel_hip_server_create": (_st, [ctypes.POINTER(ServeConfig), ctypes.POINTER(_p)]),
     "axhel_hip_server_destroy": (_st, [_p]),
@@ -442,6 +443,12 @@ class RNS:
     def ntt(self, result, operand, count, inverse=False, output_mod_factor=1, stream=None):
         _check(_load().axhel_hip_rns_ntt(self._h, _ptr(result), _ptr(operand), count, 1 if inverse else 0,
                                          output_mod_factor, _stream_ptr(stream)), "rns_ntt")
+
+    def ntt_noalloc(self, result, operand, count, inverse=False, output_mod_factor=1, stream=None):
+        """ntt() through the entry point that allocates nothing (tables in the
+        library's fixed per-GPU scratch array; calls overlapping on one GPU race)."""
+        _check(_load().axhel_hip_rns_ntt_noalloc(self._h, _ptr(result), _ptr(operand), count, 1 if inverse else 0,
+                                                 output_mod_factor, _stream_ptr(stream)), "rns_ntt_noalloc")
 
     def eltwise(self, op, result, operand1, operand2=None, scalars=None, count=1, input_mod_factor=1,
                 output_mod_factor=1, stream=None):

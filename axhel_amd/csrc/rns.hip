@@ -21,6 +21,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "eltwise_host.hpp"
@@ -210,6 +211,41 @@ axhel_hip_status refuse_capture(hipStream_t st) {
     return AXHEL_HIP_OK;
 }
 
+// ---- no-allocation form of the NTT call
+//
+// The job and stack tables of axhel_hip_rns_ntt_noalloc live in this
+// file-scope device array instead of a per-stream buffer the context
+// allocates.  Each GPU holds its own copy until the process ends.  In host
+// code `scratch` names a host-side stand-in, never the device memory: the
+// device address comes from hipGetSymbolAddress, taken once per device with
+// that device current, and only that address is passed to kernels.  Every
+// call on a GPU writes the same array, so calls that overlap on one GPU
+// (concurrent streams, several threads) race.
+constexpr size_t kScratchEntries = AXHEL_HIP_RNS_NOALLOC_MAX_ENTRIES;
+static __device__ __align__(256) NttJobDev scratch[kScratchEntries];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+NttJobDev *g_scratch_dev[kMaxDevices];
+
+// The scratch array's address on the current device (set-up on first use).
+hipError_t scratch_for_device(NttJobDev **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_dev[dev] = static_cast<NttJobDev *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
 // Stack layout of a transform call over count x L limbs: slots per limb of
 // the stacked ROW pass (0: no stack table), the same rule as the batches
 // (padding each limb's polynomials to whole stacks leaves at most one slot
@@ -284,6 +320,10 @@ axhel_hip_status axhel_hip_rns_create(uint64_t degree, const uint64_t *moduli, c
     if (st == AXHEL_HIP_OK) {
         hipError_t e = hipMalloc(&c->d_info, limbs * sizeof(RnsLimbDev));
         if (e == hipSuccess) e = hipMemcpy(c->d_info, info.data(), limbs * sizeof(RnsLimbDev), hipMemcpyHostToDevice);
+        // set-up of the no-allocation call on this device: the scratch
+        // array's address and the degree's kernel attributes
+        NttJobDev *scr = nullptr;
+        if (e == hipSuccess) e = scratch_for_device(&scr);
         if (e == hipSuccess) e = prepare_config(config_for_single(c->logn));
         if (e == hipSuccess) e = prepare_config(config_for(c->logn));
         if (e != hipSuccess) {
@@ -351,6 +391,35 @@ axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const u
         tab_buf->slots = slots_per_limb;
     }
     AXHEL_HIP_TRY(e);
+    return AXHEL_HIP_OK;
+}
+
+axhel_hip_status axhel_hip_rns_ntt_noalloc(axhel_hip_rns *ctx, uint64_t *result, const uint64_t *operand,
+                                           uint64_t count, int inverse, uint64_t output_mod_factor, void *stream) {
+    if (!ctx || (inverse != 0 && inverse != 1)) return AXHEL_HIP_INVALID_ARGUMENT;
+    if (count == 0) return AXHEL_HIP_OK;
+    if (!device_ptr(result) || !device_ptr(operand) || count > 0x7FFFFFFFull / ctx->limbs)
+        return AXHEL_HIP_INVALID_ARGUMENT;
+    const bool normalize = inverse ? output_mod_factor != 2 : output_mod_factor != 4;
+    const hipStream_t st = as_stream(stream);
+    if (const axhel_hip_status cs = refuse_capture(st)) return cs;
+    const uint32_t cnt = (uint32_t)count, L = ctx->limbs;
+    const uint64_t nj = (uint64_t)cnt * L;
+    const NttConfig &cfg = config_for_launch(ctx->logn, nj);
+    const uint32_t slots_per_limb = stack_slots_per_limb(cfg, cnt, L);
+    const uint64_t entries = nj + (uint64_t)slots_per_limb * L;
+    if (entries > kScratchEntries) {
+        set_last_error_text("rns_ntt_noalloc: count x limbs needs " + std::to_string(entries) +
+                            " job descriptors, above the scratch maximum of " + std::to_string(kScratchEntries));
+        return AXHEL_HIP_INVALID_ARGUMENT;
+    }
+    NttJobDev *scr = nullptr;
+    AXHEL_HIP_TRY(scratch_for_device(&scr));
+    // kernel attributes were raised at set-up for the selected configurations;
+    // this only finds them recorded
+    AXHEL_HIP_TRY(prepare_config(cfg));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    AXHEL_HIP_TRY(enqueue_rns_ntt(ctx, cfg, scr, true, result, operand, cnt, inverse, normalize, slots_per_limb, st));
     return AXHEL_HIP_OK;
 }
 

@@ -350,6 +350,22 @@ const axhel_hip_plan *axhel_hip_rns_plan(const axhel_hip_rns *ctx, uint32_t limb
    4 / 2, normalized otherwise).  result may equal operand. */
 axhel_hip_status axhel_hip_rns_ntt(axhel_hip_rns *ctx, uint64_t *result, const uint64_t *operand, uint64_t count,
                                    int inverse, uint64_t output_mod_factor, void *stream);
+/* axhel_hip_rns_ntt that allocates nothing.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory on the
+   call path (code under graph capture elsewhere in the program, tight
+   serving loops, harnesses that forbid hipMalloc).  The job and stack
+   tables go to a fixed device array of AXHEL_HIP_RNS_NOALLOC_MAX_ENTRIES
+   descriptors (1 MiB per GPU, held until the process ends) instead of a
+   buffer the context allocates; a call needing more (count x limbs
+   descriptors, doubled when the ROW pass runs stacked) returns
+   AXHEL_HIP_INVALID_ARGUMENT and writes nothing.  The array's address is
+   taken at axhel_hip_rns_create (or on the first call on a device).  Every
+   call on one GPU uses the same array: calls that overlap on a GPU (other
+   streams, other threads, other contexts) race and must be ordered by the
+   caller.  Same results as axhel_hip_rns_ntt. */
+#define AXHEL_HIP_RNS_NOALLOC_MAX_ENTRIES 16384
+axhel_hip_status axhel_hip_rns_ntt_noalloc(axhel_hip_rns *ctx, uint64_t *result, const uint64_t *operand,
+                                           uint64_t count, int inverse, uint64_t output_mod_factor, void *stream);
 /* Element-wise op over every limb with that limb's modulus: op is
    AXHEL_HIP_OP_ADD / SUB (operand2, or NULL and scalars[limbs] on the host
    for the vector-scalar forms, eltwise-add-mod.hpp:17 and
