@@ -4,11 +4,15 @@
   axhel_amd/lib/libaxhel.so      the unipi::axhel C++ drop-in API over the C ABI
   axhel_amd/lib/axhel_cxx_check  a C++ program exercising the drop-in API
   axhel_amd/lib/axhel_serve_bench  the serving-loop load driver (Poisson arrivals)
+  axhel_amd/lib/cmake/AXHEL/     the CMake package (find_package(AXHEL), AXHEL::axhel)
+  axhel_amd/lib/axhel_cxx_check_cmake  the same check program built by CMake
+                                 through that package (axhel_amd/cmake/example)
 
 Usage: python -m axhel_amd.build [-v] [-j N]
 """
 import concurrent.futures
 import os
+import shutil
 import subprocess
 import sys
 
@@ -55,15 +59,16 @@ def build(verbose=False, jobs=None):
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile_hip(s, verbose), sources))
     hip_lib = os.path.join(LIBDIR, "libaxhel_hip.so")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", hip_lib + ".tmp"] + objs + ["-lpthread", "-ldl"],
-         verbose)
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-Wl,-soname,libaxhel_hip.so", "-o", hip_lib + ".tmp"]
+         + objs + ["-lpthread", "-ldl"], verbose)
     os.replace(hip_lib + ".tmp", hip_lib)
 
     # C++ drop-in API: plain C++ over the C ABI (no HIP headers).
     cxx_lib = os.path.join(LIBDIR, "libaxhel.so")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
           "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "cxx_api.cpp"),
-          "-L", LIBDIR, "-laxhel_hip", "-Wl,-rpath,$ORIGIN", "-o", cxx_lib + ".tmp"], verbose)
+          "-L", LIBDIR, "-laxhel_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-soname,libaxhel.so", "-o", cxx_lib + ".tmp"],
+         verbose)
     os.replace(cxx_lib + ".tmp", cxx_lib)
 
     check = os.path.join(LIBDIR, "axhel_cxx_check")
@@ -79,6 +84,21 @@ def build(verbose=False, jobs=None):
           "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "serve_bench.hip"),
           "-L", LIBDIR, "-laxhel_hip", "-lpthread", "-Wl,-rpath,$ORIGIN", "-o", sb + ".tmp"], verbose)
     os.replace(sb + ".tmp", sb)
+
+    # CMake package beside the libraries, and a consumer built through it
+    pkg = os.path.join(LIBDIR, "cmake", "AXHEL")
+    os.makedirs(pkg, exist_ok=True)
+    for f in ("AXHELConfig.cmake", "AXHELConfigVersion.cmake"):
+        shutil.copyfile(os.path.join(HERE, "cmake", f + ".in"), os.path.join(pkg, f))
+    bdir = os.path.join(OBJDIR, "cmake_example")
+    shutil.rmtree(bdir, ignore_errors=True)
+    _run(["cmake", "-S", os.path.join(HERE, "cmake", "example"), "-B", bdir, "-DCMAKE_BUILD_TYPE=Release",
+          f"-DCMAKE_PREFIX_PATH={HERE}", "-DCMAKE_CXX_COMPILER=g++"], verbose)
+    _run(["cmake", "--build", bdir, "-j", "4"], verbose)
+    ck = os.path.join(LIBDIR, "axhel_cxx_check_cmake")
+    shutil.copyfile(os.path.join(bdir, "axhel_cxx_check_cmake"), ck + ".tmp")
+    os.chmod(ck + ".tmp", 0o755)
+    os.replace(ck + ".tmp", ck)
     return hip_lib
 
 

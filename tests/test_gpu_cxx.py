@@ -1,5 +1,6 @@
 """The unipi::axhel C++ drop-in (libaxhel.so) driven by a C++ program with
 host std::vector buffers, checked against the oracle."""
+import os
 import subprocess
 
 import numpy as np
@@ -12,14 +13,21 @@ from tests.helpers import ntt_params
 pytestmark = pytest.mark.gpu
 
 
+CMAKE_CHECK_PATH = os.path.join(ax.LIB_DIR, "axhel_cxx_check_cmake")
+
+
 @pytest.mark.parametrize("n,bits", [(4096, 50), (8, 62), (65536, 55)])
-def test_cxx_dropin_program(n, bits):
+@pytest.mark.parametrize("exe", ["g++", "cmake"])
+def test_cxx_dropin_program(n, bits, exe):
+    """exe: the program linked by hand-written g++ flags, or built by CMake
+    through find_package(AXHEL) and AXHEL::axhel (axhel_amd/cmake/example)."""
     q, psi = ntt_params(n, bits)
     mt = O.MT19937_64(0)
     a = mt.random_vector(n, q)
     b = mt.random_vector(n, q)
     stdin = f"{n} {q} {psi}\n" + " ".join(map(str, a.tolist())) + "\n" + " ".join(map(str, b.tolist())) + "\n"
-    r = subprocess.run([ax.CXX_CHECK_PATH], input=stdin, capture_output=True, text=True, timeout=120)
+    path = ax.CXX_CHECK_PATH if exe == "g++" else CMAKE_CHECK_PATH
+    r = subprocess.run([path], input=stdin, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     out = {}
     for line in r.stdout.splitlines():
