@@ -30,6 +30,7 @@ NO_DEVICE = 5
 REJECTED = 6
 TIMEOUT = 7
 CANCELLED = 8
+PENDING = 9
 
 
 class AxhelError(RuntimeError):
@@ -520,7 +521,8 @@ class Server:
 
     submit() takes one request over `limbs` vectors of `degree` words (limb l
     at offset l*degree of every buffer) and returns a ticket; wait(ticket)
-    returns the final status code (OK, TIMEOUT, CANCELLED, NO_MEMORY, ...)."""
+    returns the final status code (OK, TIMEOUT, CANCELLED, NO_MEMORY, ...),
+    or PENDING when wait's own timeout passed first (the ticket stays valid)."""
 
     def __init__(self, memory_budget_bytes=0, max_batch_limbs=0, slots=0, backend=BACKEND_DEVICE,
                  null_latency_us=0, fail_every_alloc=0, fail_every_batch=0):

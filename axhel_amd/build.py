@@ -4,6 +4,8 @@
   axhel_amd/lib/libaxhel.so      the unipi::axhel C++ drop-in API over the C ABI
   axhel_amd/lib/axhel_cxx_check  a C++ program exercising the drop-in API
   axhel_amd/lib/axhel_serve_bench  the serving-loop load driver (Poisson arrivals)
+  axhel_amd/lib/libaxhel_arith_hooks.so  test hooks: each device arithmetic
+                                 primitive applied element-wise (tests only)
   axhel_amd/lib/cmake/AXHEL/     the CMake package (find_package(AXHEL), AXHEL::axhel)
   axhel_amd/lib/axhel_cxx_check_cmake  the same check program built by CMake
                                  through that package (axhel_amd/cmake/example)
@@ -57,11 +59,16 @@ def build(verbose=False, jobs=None):
     jobs = jobs or min(8, os.cpu_count() or 4)
     sources = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        hooks_obj = ex.submit(_compile_hip, "arith_hooks.hip", verbose)
         objs = list(ex.map(lambda s: _compile_hip(s, verbose), sources))
+        hooks_obj = hooks_obj.result()
     hip_lib = os.path.join(LIBDIR, "libaxhel_hip.so")
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-Wl,-soname,libaxhel_hip.so", "-o", hip_lib + ".tmp"]
          + objs + ["-lpthread", "-ldl"], verbose)
     os.replace(hip_lib + ".tmp", hip_lib)
+    hooks_lib = os.path.join(LIBDIR, "libaxhel_arith_hooks.so")
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", hooks_lib + ".tmp", hooks_obj], verbose)
+    os.replace(hooks_lib + ".tmp", hooks_lib)
 
     # C++ drop-in API: plain C++ over the C ABI (no HIP headers).
     cxx_lib = os.path.join(LIBDIR, "libaxhel.so")

@@ -198,6 +198,7 @@ const char *axhel_hip_status_string(axhel_hip_status s) {
         case AXHEL_HIP_NO_DEVICE: return "no GPU device";
         case AXHEL_HIP_REJECTED: return "request rejected";
         case AXHEL_HIP_TIMEOUT: return "timed out";
+        case AXHEL_HIP_PENDING: return "still pending";
         case AXHEL_HIP_CANCELLED: return "cancelled";
     }
     return "unknown status";

@@ -660,7 +660,7 @@ axhel_hip_status axhel_hip_server_wait(axhel_hip_server *s, uint64_t ticket, int
     else done = s->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), pred);
     --s->active_waiters;
     s->cv_done.notify_all();
-    if (!done) return AXHEL_HIP_TIMEOUT;
+    if (!done) return AXHEL_HIP_PENDING;
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) return AXHEL_HIP_INVALID_ARGUMENT;  // collected by another waiter
     Req *R = it->second;

@@ -35,8 +35,9 @@ typedef enum axhel_hip_status {
     AXHEL_HIP_HIP_ERROR = 4,
     AXHEL_HIP_NO_DEVICE = 5,
     AXHEL_HIP_REJECTED = 6,  /* serving: admission refused (memory budget / shutdown) */
-    AXHEL_HIP_TIMEOUT = 7,
-    AXHEL_HIP_CANCELLED = 8
+    AXHEL_HIP_TIMEOUT = 7,   /* serving: a request dropped because it was not started by its deadline */
+    AXHEL_HIP_CANCELLED = 8,
+    AXHEL_HIP_PENDING = 9    /* serving: axhel_hip_server_wait gave up first; the request runs on */
 } axhel_hip_status;
 
 /* Human-readable text of a status, and the last HIP error string seen by the
@@ -304,10 +305,12 @@ axhel_hip_status axhel_hip_server_plan(axhel_hip_server *server, uint64_t degree
 axhel_hip_status axhel_hip_server_submit(axhel_hip_server *server, const axhel_hip_request *request,
                                          uint64_t *ticket);
 /* Waits up to timeout_us (< 0: forever) for the request; on completion
-   returns its final status and forgets the ticket.  AXHEL_HIP_TIMEOUT here
-   means "still running" (the ticket stays valid).  When several threads wait
-   on one ticket, one of them collects the status and the others return
-   AXHEL_HIP_INVALID_ARGUMENT. */
+   returns its final status (AXHEL_HIP_OK, or AXHEL_HIP_TIMEOUT for a request
+   dropped at its deadline, AXHEL_HIP_CANCELLED, AXHEL_HIP_NO_MEMORY, ...) and
+   forgets the ticket.  AXHEL_HIP_PENDING means the wait gave up first: the
+   request is still queued or running and the ticket stays valid.  When
+   several threads wait on one ticket, one of them collects the status and
+   the others return AXHEL_HIP_INVALID_ARGUMENT. */
 axhel_hip_status axhel_hip_server_wait(axhel_hip_server *server, uint64_t ticket, int64_t timeout_us);
 /* Cancels a queued request (AXHEL_HIP_OK), or reports AXHEL_HIP_REJECTED if
    it already runs or finished. */

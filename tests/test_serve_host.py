@@ -97,7 +97,7 @@ def test_queue_timeout_and_cancel():
 def test_wait_timeout_keeps_ticket():
     s = ax.Server(backend=ax.BACKEND_NULL, null_latency_us=80000, slots=1, memory_budget_bytes=64 * MIB)
     st, t, k = req(s)
-    assert s.wait(t, timeout_us=1000) == ax.TIMEOUT  # still running
+    assert s.wait(t, timeout_us=1000) == ax.PENDING  # still running
     assert s.wait(t) == ax.OK
     assert s.wait(t) == ax.INVALID_ARGUMENT  # ticket released after completion
     s.close()
