@@ -117,9 +117,10 @@ __device__ __forceinline__ uint64_t twice_plus(uint64_t a, uint64_t b) {
     return r;
 }
 
-// Per-limb constants of the butterflies: q, 2q, 4q and their negations mod 2^64.
+// Per-limb constants of the butterflies: q, 2q, 4q, 8q and their negations
+// mod 2^64 (4q and 8q serve the relaxed schedule, whose moduli are below 2^60).
 struct ModConsts {
-    uint64_t q, q2, nq, nq2, q4, nq4;
+    uint64_t q, q2, nq, nq2, q4, nq4, q8, nq8;
 };
 
 __device__ __forceinline__ ModConsts mod_consts(uint64_t q) {
@@ -130,6 +131,8 @@ __device__ __forceinline__ ModConsts mod_consts(uint64_t q) {
     m.nq2 = 0 - m.q2;
     m.q4 = q << 2;
     m.nq4 = 0 - m.q4;
+    m.q8 = q << 3;
+    m.nq8 = 0 - m.q8;
     return m;
 }
 
@@ -164,26 +167,28 @@ __device__ __forceinline__ void fwd_bfly(uint64_t &x, uint64_t &y, uint64_t w, u
 }
 
 // Forward butterfly of the relaxed schedule, for transforms whose output is
-// normalized to [0, q) and whose modulus is below 2^61.  A normalized output
-// is the unique canonical residue, so any congruent intermediate values give
-// the reference's result bit for bit (for inputs in its domain [0, 4q),
-// ntt-native.cpp:17); the schedule trades the per-stage bound for fewer
-// conditional subtractions:
-//   SKIP (odd local stage):  x' = x            x < 6q  -> outputs < 8q
-//   SUB4 (even local stage): x' = CS(x, 4q)    x < 8q  -> x' < 4q, outputs < 6q
-//   LAST (bit 0, last pass): x' = CS(CS(x,4q),2q)       -> outputs < 4q
-// Every stage reads x' + t and x' + 2q - t with t = Shoup(y) in [0, 2q) for
-// any 64-bit y, and no two SKIP stages are adjacent (bit 0 of every pass is
-// even), so every word stays below 8q < 2^64.  LAST leaves the [0, 4q) range
-// the normalization of the exact schedule expects.
-enum class FwdStage { SKIP, SUB4, LAST };
+// normalized to [0, q) and whose moduli are all below 2^60.  A normalized
+// output is the unique canonical residue, so any congruent intermediate
+// values give the reference's result bit for bit (for inputs in its domain
+// [0, 4q), ntt-native.cpp:17); the schedule lets words grow to 16q and trades
+// the per-stage bound for fewer conditional subtractions.  Every stage reads
+// x' + t and x' + 2q - t with t = Shoup(y) in [0, 2q) for any 64-bit y, so a
+// stage without subtraction raises the bound on its words by 2q:
+//   SKIP: x' = x                        x < 14q -> outputs < 16q
+//   SUB8: x' = CS(x, 8q)                x < 16q -> x' < 8q, outputs < 10q
+//   LAST: x' = CS(CS(CS(x,8q),4q),2q)   x < 16q -> x' < 2q, outputs < 4q
+// The passes place one SUB8 in every four stages (a first pass starts from
+// inputs below 4q and skips its first six), which keeps every word below
+// 16q < 2^64; LAST, the transform's bit 0, leaves the [0, 4q) range the
+// normalization of the exact schedule expects (stage_kind in ntt_kernels.hpp).
+enum class FwdStage { SKIP, SUB8, LAST };
 
 template <FwdStage S>
 __device__ __forceinline__ void fwd_bfly_relaxed(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wq,
                                                  const ModConsts &m) {
     uint64_t xh = x;
-    if constexpr (S != FwdStage::SKIP) xh = csub_n(xh, m.q4, m.nq4);
-    if constexpr (S == FwdStage::LAST) xh = csub_n(xh, m.q2, m.nq2);
+    if constexpr (S != FwdStage::SKIP) xh = csub_n(xh, m.q8, m.nq8);
+    if constexpr (S == FwdStage::LAST) xh = csub_n(csub_n(xh, m.q4, m.nq4), m.q2, m.nq2);
     const uint64_t sum = shoup_acc(xh, y, w, wq, m.nq);
     y = sub64(twice_plus(xh, m.q2), sum);
     x = sum;

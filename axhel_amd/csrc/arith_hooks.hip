@@ -26,7 +26,7 @@ enum Op {
     OP_TWICE_PLUS = 4,      // out0 = 2a + b mod 2^64
     OP_FWD_BFLY = 5,        // (out0, out1) = fwd_bfly(x = a, y = b, w = c, wq = d, q = e)
     OP_FWD_SKIP = 6,        // fwd_bfly_relaxed<SKIP>, same operands
-    OP_FWD_SUB4 = 7,        // fwd_bfly_relaxed<SUB4>
+    OP_FWD_SUB8 = 7,        // fwd_bfly_relaxed<SUB8>
     OP_FWD_LAST = 8,        // fwd_bfly_relaxed<LAST>
     OP_INV_BFLY = 9,        // inv_bfly(x = a, y = b, w = c, wq = d, q = e)
     OP_INV_FINAL = 10,      // inv_final_bfly(x = a, y = b, ni = c, niq = d, s = f, sq = g, q = e)
@@ -52,7 +52,7 @@ __global__ void hooks_kernel(int op, const uint64_t *in, uint64_t *out, uint64_t
         case OP_TWICE_PLUS: r0 = twice_plus(a, b); break;
         case OP_FWD_BFLY: fwd_bfly(a, b, c, d, m); r0 = a; r1 = b; break;
         case OP_FWD_SKIP: fwd_bfly_relaxed<FwdStage::SKIP>(a, b, c, d, m); r0 = a; r1 = b; break;
-        case OP_FWD_SUB4: fwd_bfly_relaxed<FwdStage::SUB4>(a, b, c, d, m); r0 = a; r1 = b; break;
+        case OP_FWD_SUB8: fwd_bfly_relaxed<FwdStage::SUB8>(a, b, c, d, m); r0 = a; r1 = b; break;
         case OP_FWD_LAST: fwd_bfly_relaxed<FwdStage::LAST>(a, b, c, d, m); r0 = a; r1 = b; break;
         case OP_INV_BFLY: inv_bfly(a, b, c, d, m); r0 = a; r1 = b; break;
         case OP_INV_FINAL: inv_final_bfly(a, b, c, d, f, g, m); r0 = a; r1 = b; break;

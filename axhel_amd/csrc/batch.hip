@@ -31,7 +31,7 @@ struct axhel_hip_batch {
         uint32_t count;
         size_t offset;  // first descriptor in d_jobs
         bool vec16;     // all src/dst 16-byte aligned
-        bool relax_ok;  // every modulus below 2^61 (relaxed forward schedule allowed)
+        bool relax_ok;  // every modulus below 2^60 (relaxed forward schedule allowed)
         // stacked ROW pass: stack table at d_jobs + stack_offset (stack_gpw 0: none)
         size_t stack_offset;
         uint32_t stacks;

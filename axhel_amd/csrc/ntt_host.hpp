@@ -30,7 +30,7 @@ constexpr uint32_t kMaxLogN = 17;
 struct PassKernel {
     const void *fn = nullptr;
     // forward kernels: the same geometry on the relaxed butterfly schedule
-    // (normalized outputs only, every modulus below 2^61); null otherwise
+    // (normalized outputs only, every modulus below 2^60); null otherwise
     const void *fn_relaxed = nullptr;
     int log_g = 0;
     int log_e = 0;
@@ -102,7 +102,7 @@ axhel_dev::NttJobDev make_job(const axhel_hip_plan *p, const uint64_t *src, uint
 // aligned (enables 16-byte global accesses).  stacks: the same jobs as a
 // stack table; the ROW pass of a two-pass configuration then runs stacked
 // when the table was built for that pass's group count.  relax_ok: every
-// job's modulus is below 2^61 (relax_ok_modulus), so a normalized forward may
+// job's modulus is below 2^60 (relax_ok_modulus), so a normalized forward may
 // run on the relaxed butterfly schedule (same outputs, fewer subtractions).
 hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool normalize,
                       const axhel_dev::NttJobDev *jobs, const axhel_dev::NttJobDev &inline_job, uint32_t count,
@@ -110,8 +110,8 @@ hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool no
                       const StackTable *stacks = nullptr, bool relax_ok = false);
 
 // Moduli for which the relaxed forward schedule keeps every word below 2^64
-// (its words stay below 8q).
-inline bool relax_ok_modulus(uint64_t q) { return q < (1ull << 61); }
+// (its words stay below 16q).
+inline bool relax_ok_modulus(uint64_t q) { return q < (1ull << 60); }
 
 // Whether a forward transform under cfg runs the relaxed schedule: its output
 // is normalized, every modulus passes relax_ok_modulus (relax_ok), and every

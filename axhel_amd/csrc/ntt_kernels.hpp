@@ -16,7 +16,7 @@
 // butterflies on the previous bit that feed them, so outputs are bit-identical
 // to the scalar loop, lazy bits included.  Only the order of independent
 // butterflies changes.  The one exception is chosen by the host for forward
-// transforms with a normalized output and every q < 2^61 (RLX = true): the
+// transforms with a normalized output and every q < 2^60 (RLX = true): the
 // conditional subtractions follow the relaxed schedule of fwd_bfly_relaxed,
 // whose intermediate words differ but whose normalized outputs, the unique
 // residues in [0, q), are the reference's for inputs in [0, 4q).
@@ -271,22 +271,22 @@ struct TwStage {
     // Entries past the end are clamped to the last one (a duplicate load,
     // not stored), keeping the loads branch-free so later loads need not
     // wait for them.
+    // With R = n >> LOG_G rows, the table index of entry i folds into one
+    // shift of a workgroup-uniform term:
+    //     forward:  (R + o - 1) << j  +  i + 1                 j = floor(log2(i + 1))
+    //     inverse:  (o + 2 - 2R) << j  +  i + n + 1 - G        j = floor(log2(G - 1 - i))
+    // (the inverse's uniform term is negative for most rows; the sum is
+    // formed modulo 2^32 and lands in [0, n)).
     template <bool FWD>
     __device__ __forceinline__ void fetch_row(const ulonglong2 *tw, uint32_t n, uint32_t o, uint32_t t) {
+        const uint32_t rows = n >> LOG_G;
+        const uint32_t c = FWD ? rows + o - 1u : o + 2u - 2u * rows;  // uniform
+        const uint32_t d = FWD ? 1u : n + 1u - G;                     // uniform
         static_for<0, K>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             const uint32_t i = min(t + (uint32_t)k * NT, G - 2u);
-            uint32_t j, lp, base;
-            if constexpr (FWD) {
-                j = 31u - __clz(i + 1u);
-                lp = i + 1u - (1u << j);
-                base = n >> (LOG_G - j);
-            } else {
-                j = 31u - __clz(G - i - 1u);
-                lp = i - (G - (2u << j));
-                base = n + 1u - (n >> (LOG_G - 1 - j));
-            }
-            v[k] = ld_g2(tw + base + (o << j) + lp);
+            const uint32_t j = 31u - __clz(FWD ? i + 1u : G - 1u - i);
+            v[k] = ld_g2(tw + ((c << j) + i + d));
         });
     }
 
@@ -310,12 +310,27 @@ struct TwStage {
     }
 };
 
+// Stage kind of local bit p of a forward pass on the relaxed schedule
+// (fwd_bfly_relaxed), s = LOG_G - 1 - p the stage's position in the pass:
+//   COL pass (always the transform's first): inputs below 4q, so stages 0-5
+//     skip the subtraction (words reach 16q), then one SUB8 in every four
+//     (s = 6, 10, ...); it hands the next pass words below 16q;
+//   ROW pass (second pass, or the only one): inputs below 16q, SUB8 at
+//     s = 0, 4, 8, ..., and its bit 0, the transform's last stage, is LAST.
+// Between two subtractions at most three SKIP stages follow a SUB8 (10q ->
+// 16q), so every word stays below 16q.
+template <int LOG_G, bool COL>
+__device__ __forceinline__ constexpr FwdStage stage_kind(int p) {
+    const int s = LOG_G - 1 - p;
+    if (!COL && p == 0) return FwdStage::LAST;
+    if (COL) return (s >= 6 && (s - 6) % 4 == 0) ? FwdStage::SUB8 : FwdStage::SKIP;
+    return s % 4 == 0 ? FwdStage::SUB8 : FwdStage::SKIP;
+}
+
 // The butterfly chunks of one pass over registers v (chunk-0 layout on entry,
 // last-chunk layout on exit), redistributing through LDS between chunks.
 // Every thread of the workgroup must call it (it holds barriers).
-// RLX: forward butterflies on the relaxed schedule of fwd_bfly_relaxed (local
-// bit p odd: SKIP; even: SUB4; bit 0 of a ROW or single pass, the transform's
-// last stage: LAST).
+// RLX: forward butterflies on the relaxed schedule (stage_kind).
 template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, bool RLX = false, typename TW>
 __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *lds, uint32_t g, uint32_t tid,
                                            const ModConsts &m, const TW &tws, bool final_inv, const NttJobDev &J) {
@@ -366,9 +381,7 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
                         }
                     }
                     if constexpr (FWD && RLX) {
-                        constexpr FwdStage S = (p & 1)             ? FwdStage::SKIP
-                                               : (!COL && p == 0) ? FwdStage::LAST
-                                                                  : FwdStage::SUB4;
+                        constexpr FwdStage S = stage_kind<LOG_G, COL>(p);
                         fwd_bfly_relaxed<S>(v[r], v[r | bit], w[rp], wq[rp], m);
                     } else if constexpr (FWD) {
                         fwd_bfly(v[r], v[r | bit], w[rp], wq[rp], m);
@@ -661,15 +674,14 @@ ntt_row_stack_kernel(PassArgs a) {
             v[r] = lds[L::idx2(g, k0, Gm::kr(0, r))];
         });
     } else {
+        // an empty slot reads slot 0's row (always a job) and stores nothing:
+        // no separate path for its registers
+        const uint64_t *lsrc = valid ? src : (a.first_pass ? J0.src : J0.dst);
         const uint32_t k0 = Gm::kt(0, tid);
-        if (valid) {
-            static_for<0, E>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                v[r] = ld_g(src + row + k0 + Gm::kr(0, r));
-            });
-        } else {
-            static_for<0, E>([&](auto rc) { v[decltype(rc)::value] = 0; });
-        }
+        static_for<0, E>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            v[r] = ld_g(lsrc + row + k0 + Gm::kr(0, r));
+        });
         tstage.store(tw_lds, t);
         __syncthreads();  // publishes the twiddles
     }

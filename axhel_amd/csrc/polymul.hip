@@ -30,7 +30,7 @@ struct axhel_hip_polymul {
         size_t off_elt_mul, off_elt_fma;  // count EltLimbDev each, unfused path
         bool vec16;
         bool has_addend;
-        bool relax_ok;  // every modulus below 2^61: the forward transforms may run relaxed
+        bool relax_ok;  // every modulus below 2^60: the forward transforms may run relaxed
         // fused path over stacks of same-plan jobs: stack table at off_fstack
         // (fstack_gpw 0: none)
         size_t off_fstack;

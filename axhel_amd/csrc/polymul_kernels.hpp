@@ -13,7 +13,7 @@
 // arithmetic on the same words, so the result is bit-identical to running
 // the five calls one after another.  RLX = true runs the two forward
 // transforms on the relaxed schedule of fwd_bfly_relaxed (every modulus
-// below 2^61): their words differ on the way, but the normalised NTT(a) and
+// below 2^60): their words differ on the way, but the normalised NTT(a) and
 // NTT(b) the product consumes are the unique residues, so the output is
 // unchanged.
 #pragma once

@@ -49,7 +49,7 @@ struct axhel_hip_rns {
     uint32_t limbs = 0;
     std::vector<axhel_hip_plan *> plans;
     RnsLimbDev *d_info = nullptr;
-    bool relax_ok = true;  // every modulus below 2^61
+    bool relax_ok = true;  // every modulus below 2^60
     // job-table buffer per stream, and the NTT call whose table it holds
     struct StreamTable {
         hipStream_t stream;

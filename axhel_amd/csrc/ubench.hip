@@ -11,6 +11,7 @@
 #include "../../include/axhel_hip_ubench.h"
 #include "arith.hpp"
 #include "common.hpp"
+#include "ntt_kernels.hpp"
 
 namespace {
 
@@ -26,21 +27,27 @@ __device__ __forceinline__ void pass_stages(uint64_t &x, uint64_t &y, uint64_t w
     if constexpr (S == AXHEL_HIP_UBENCH_FWD_EXACT) {
 #pragma unroll
         for (int s = 0; s < kStages; ++s) axhel_dev::fwd_bfly(x, y, w, wq, m);
-    } else if constexpr (S == AXHEL_HIP_UBENCH_FWD_RELAXED || S == AXHEL_HIP_UBENCH_FWD_RELAXED_LAST) {
-        // local bits 7 .. 0: odd bits skip the subtraction, even bits subtract 4q;
-        // the last pass's bit 0 runs LAST and the output is normalized
+    } else if constexpr (S == AXHEL_HIP_UBENCH_FWD_RELAXED) {
+        // the stage kinds of an 8-bit first (COL) pass
+#pragma unroll
+        for (int p = kStages - 1; p >= 0; --p) {
+            if (axhel_dev::stage_kind<kStages, true>(p) == FwdStage::SUB8)
+                axhel_dev::fwd_bfly_relaxed<FwdStage::SUB8>(x, y, w, wq, m);
+            else
+                axhel_dev::fwd_bfly_relaxed<FwdStage::SKIP>(x, y, w, wq, m);
+        }
+    } else if constexpr (S == AXHEL_HIP_UBENCH_FWD_RELAXED_LAST) {
+        // the stage kinds of an 8-bit last (ROW) pass, then the normalization
 #pragma unroll
         for (int p = kStages - 1; p >= 1; --p) {
-            if (p & 1) axhel_dev::fwd_bfly_relaxed<FwdStage::SKIP>(x, y, w, wq, m);
-            else axhel_dev::fwd_bfly_relaxed<FwdStage::SUB4>(x, y, w, wq, m);
+            if (axhel_dev::stage_kind<kStages, false>(p) == FwdStage::SUB8)
+                axhel_dev::fwd_bfly_relaxed<FwdStage::SUB8>(x, y, w, wq, m);
+            else
+                axhel_dev::fwd_bfly_relaxed<FwdStage::SKIP>(x, y, w, wq, m);
         }
-        if constexpr (S == AXHEL_HIP_UBENCH_FWD_RELAXED) {
-            axhel_dev::fwd_bfly_relaxed<FwdStage::SUB4>(x, y, w, wq, m);
-        } else {
-            axhel_dev::fwd_bfly_relaxed<FwdStage::LAST>(x, y, w, wq, m);
-            x = axhel_dev::csub_n(axhel_dev::csub_n(x, m.q2, m.nq2), m.q, m.nq);
-            y = axhel_dev::csub_n(axhel_dev::csub_n(y, m.q2, m.nq2), m.q, m.nq);
-        }
+        axhel_dev::fwd_bfly_relaxed<FwdStage::LAST>(x, y, w, wq, m);
+        x = axhel_dev::csub_n(axhel_dev::csub_n(x, m.q2, m.nq2), m.q, m.nq);
+        y = axhel_dev::csub_n(axhel_dev::csub_n(y, m.q2, m.nq2), m.q, m.nq);
     } else if constexpr (S == AXHEL_HIP_UBENCH_INV) {
 #pragma unroll
         for (int s = 0; s < kStages; ++s) axhel_dev::inv_bfly(x, y, w, wq, m);

@@ -15,11 +15,12 @@ extern "C" {
    is the stage sequence of one 8-bit pass:
      FWD_EXACT          8 forward Harvey butterflies (ntt-internal.hpp:20-31):
                         every forward pass of a lazy transform, and of any
-                        transform whose modulus is 2^61 or more;
-     FWD_RELAXED        relaxed forward schedule, local bits 7..0 alternating
-                        no subtraction / subtract 4q: the COL pass of a
-                        normalized forward transform;
-     FWD_RELAXED_LAST   the same with bit 0 on the final-stage form and both
+                        transform whose modulus is 2^60 or more;
+     FWD_RELAXED        relaxed forward schedule of a first pass: six stages
+                        without subtraction, one subtracting 8q, one without:
+                        the COL pass of a normalized forward transform;
+     FWD_RELAXED_LAST   relaxed schedule of a last pass: 8q subtracted at its
+                        stages 0 and 4, bit 0 on the final-stage form and both
                         words normalized to [0, q): the ROW pass of a
                         normalized forward transform;
      INV                8 inverse butterflies (ntt-internal.hpp:37-49): the

@@ -19,7 +19,7 @@ from tests.helpers import MODULUS_BITS, boundary_values
 pytestmark = pytest.mark.gpu
 
 M64 = (1 << 64) - 1
-(MULHI64, SHOUP_ACC, CSUB_N, SUB64, TWICE_PLUS, FWD, FWD_SKIP, FWD_SUB4, FWD_LAST, INV, INV_FINAL, BARRETT_MUL,
+(MULHI64, SHOUP_ACC, CSUB_N, SUB64, TWICE_PLUS, FWD, FWD_SKIP, FWD_SUB8, FWD_LAST, INV, INV_FINAL, BARRETT_MUL,
  BARRETT_REDUCE) = range(13)
 
 _hooks = None
@@ -116,8 +116,8 @@ def fwd_ref(x, y, w, wq, q, pre):
     return (xh + t) & M64, (xh + 2 * q - t) & M64
 
 
-PRE = {FWD: lambda x, q: cs(x, 2 * q), FWD_SKIP: lambda x, q: x, FWD_SUB4: lambda x, q: cs(x, 4 * q),
-       FWD_LAST: lambda x, q: cs(cs(x, 4 * q), 2 * q)}
+PRE = {FWD: lambda x, q: cs(x, 2 * q), FWD_SKIP: lambda x, q: x, FWD_SUB8: lambda x, q: cs(x, 8 * q),
+       FWD_LAST: lambda x, q: cs(cs(cs(x, 8 * q), 4 * q), 2 * q)}
 
 
 def test_forward_butterflies_equal_the_harvey_formula():
@@ -132,9 +132,9 @@ def test_forward_butterflies_equal_the_harvey_formula():
         items = [(x, y, wi, (wi << 64) // q, q) for wi in w[:2] for x in e for y in e]
         r = randoms(rng, 4 * 256, 1 << 64)
         items += [(r[i], r[i + 1], r[i + 2], r[i + 3], q) for i in range(0, len(r), 4)]
-        for op in (FWD, FWD_SKIP, FWD_SUB4, FWD_LAST):
-            if op != FWD and q >= 1 << 61:
-                continue  # the relaxed schedule runs only for moduli below 2^61
+        for op in (FWD, FWD_SKIP, FWD_SUB8, FWD_LAST):
+            if op != FWD and q >= 1 << 60:
+                continue  # the relaxed schedule runs only for moduli below 2^60
             got = apply(op, items)
             want = [fwd_ref(x, y, wi, wq, q, PRE[op]) for x, y, wi, wq, _ in items]
             assert got == want, (q, op)
@@ -152,11 +152,18 @@ def test_forward_butterfly_range_and_congruence():
             t = y * psi % q
             assert a < 4 * q and b < 4 * q, (q, x, y)
             assert a % q == (x + t) % q and b % q == (x - t) % q, (q, x, y)
-        if q < 1 << 61:
-            # relaxed stages: SUB4 takes x < 8q, LAST leaves [0, 4q)
-            vals8 = boundary_values(q, 8)
-            items = [(x, y, psi, wq, q) for x in vals8 for y in vals]
-            for op, bound in ((FWD_SUB4, 6 * q), (FWD_LAST, 4 * q)):
+        if q < 1 << 60:
+            # relaxed stages: SKIP takes x < 14q to below 16q; SUB8 and LAST
+            # take x < 16q to below 10q and 4q (y: any word below 16q)
+            vals16 = boundary_values(q, 16)
+            items = [(x, y, psi, wq, q) for x in vals16 for y in vals16]
+            for op, bound in ((FWD_SUB8, 10 * q), (FWD_LAST, 4 * q)):
+                for (x, y, *_), (a, b) in zip(items, apply(op, items)):
+                    t = y * psi % q
+                    assert a < bound and b < bound, (q, op, x, y)
+                    assert a % q == (x + t) % q and b % q == (x - t) % q, (q, op, x, y)
+            items = [(x, y, psi, wq, q) for x in vals16 if x < 14 * q for y in vals16]
+            for op, bound in ((FWD_SKIP, 16 * q),):
                 for (x, y, *_), (a, b) in zip(items, apply(op, items)):
                     t = y * psi % q
                     assert a < bound and b < bound, (q, op, x, y)

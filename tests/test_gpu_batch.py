@@ -206,7 +206,7 @@ def test_batch_rejects_host_pointers():
 
 def test_random_mixed_batches_equal_plain_calls():
     """Seeded random batches: degrees 2^1..2^17, moduli of 20..62 bits (both
-    sides of the 2^61 schedule gate), both directions and output ranges,
+    sides of the 2^60 schedule gate), both directions and output ranges,
     in-place and out-of-place jobs, buffers only 8-byte aligned (the batch
     then uses 8-byte accesses).  Every job equals the plain call; a sample
     equals the oracle."""

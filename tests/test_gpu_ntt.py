@@ -184,17 +184,17 @@ def test_device_generated_tables_equal_the_oracle(logn):
 
 @pytest.mark.parametrize("logn", [1, 4, 12, 13, 14, 16, 17])
 def test_relaxed_schedule_boundary_moduli(logn):
-    """A normalized forward with every modulus below 2^61 runs on the relaxed
-    butterfly schedule (intermediate words up to 8q); at or above 2^61 it runs
-    the reference's schedule.  The largest NTT prime below 2^61 with inputs at
-    the top of the domain (every word 4q - 1) and random inputs, and the
-    largest below 2^62, equal the oracle through plain calls, a batch group
-    that mixes both (exact schedule for the group), a batch group of the
-    61-bit prime alone (relaxed) and an RNS context over both."""
+    """A normalized forward with every modulus below 2^60 runs on the relaxed
+    butterfly schedule (intermediate words up to 16q); at or above 2^60 it
+    runs the reference's schedule.  The largest NTT prime below 2^60 with
+    inputs at the top of the domain (every word 4q - 1) and random inputs,
+    and the largest below 2^61, equal the oracle through plain calls, a batch
+    group that mixes both (exact schedule for the group), a batch group of
+    the 60-bit prime alone (relaxed) and an RNS context over both."""
     torch = pytest.importorskip("torch")
     n = 1 << logn
-    qs = [params.find_ntt_prime(n, 61), params.find_ntt_prime(n, 62)]
-    assert qs[0] < 1 << 61 <= qs[1]
+    qs = [params.find_ntt_prime(n, 60), params.find_ntt_prime(n, 61)]
+    assert qs[0] < 1 << 60 <= qs[1]
     psis = [params.find_primitive_2n_root(n, q) for q in qs]
     plans = [ax.NTT(n, q, p) for q, p in zip(qs, psis)]
     tabs = [O.Tables(n, q, p) for q, p in zip(qs, psis)]
@@ -220,7 +220,7 @@ def test_relaxed_schedule_boundary_moduli(logn):
     alone.launch()
     torch.cuda.synchronize()
     for k in range(2):
-        assert np.array_equal(host(out[0, k]), want[0][k]), ("61-bit batch", k)
+        assert np.array_equal(host(out[0, k]), want[0][k]), ("60-bit batch", k)
     mixed.close()
     alone.close()
     ctx = ax.RNS(n, qs, psis)

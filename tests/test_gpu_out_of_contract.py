@@ -8,7 +8,7 @@ evaluate the same formulas in the same order, so they equal the oracle bit
 for bit on such words too.
 
 One schedule is exempt: the normalized forward transform for moduli below
-2^61 assumes the stated domain [0, 4q) (ntt-native.cpp:17) and is tested at
+2^60 assumes the stated domain [0, 4q) (ntt-native.cpp:17) and is tested at
 that domain's boundary in test_gpu_ntt.py.  Here the normalized forward runs
 only with 62-bit moduli, where the kernels keep the literal formulas."""
 import numpy as np
@@ -71,7 +71,7 @@ def _check_ntt(ntt, t, n, q, rng, label):
     assert np.array_equal(host(r), O.compute_inverse(a, t, 2, 2)), ("inv lazy", label)
     ntt.ComputeInverse(r, d, 2, 1)
     assert np.array_equal(host(r), O.compute_inverse(a, t, 2, 1)), ("inv norm", label)
-    if q >= 1 << 61:  # literal formulas for the normalized forward as well
+    if q >= 1 << 60:  # literal formulas for the normalized forward as well
         ntt.ComputeForward(r, d, 4, 1)
         assert np.array_equal(host(r), O.compute_forward(a, t, 4, 1)), ("fwd norm", label)
 

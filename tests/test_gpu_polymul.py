@@ -145,7 +145,7 @@ def test_polymul_stacked_every_config(logn):
 def test_random_mixed_polymul_jobs():
     """Seeded random product jobs in one call: degrees 2^10..2^17 (single- and
     two-pass fused kernels), moduli of 30..62 bits (both sides of the relaxed
-    schedule's 2^61 gate), with and without an addend, every buffer only
+    schedule's 2^60 gate), with and without an addend, every buffer only
     8-byte aligned.  Fused equals the five-call sequence on every job; a
     sample equals the oracle."""
     torch = torch_mod()
