@@ -232,17 +232,20 @@ __device__ __forceinline__ constexpr uint32_t tw_lds_index(int j, uint32_t lp) {
     return (FWD ? (1u << j) - 1u : (1u << LOG_G) - (2u << j)) + lp;
 }
 
+// Every accessor takes the thread's part of the local index in the current
+// chunk (Geom::kt) and the compile-time register part of lp; entry
+// lp = (kt >> (P+1)) + pr.
 template <int LOG_G, bool FWD, bool COL>
 struct TwGlobal {
     const ulonglong2 *tw;
     uint32_t n, blo, o;
     template <int P>
-    __device__ __forceinline__ ulonglong2 get(uint32_t lp) const {
+    __device__ __forceinline__ ulonglong2 get(uint32_t kt, uint32_t pr) const {
         constexpr int j = LOG_G - 1 - P;
         const uint32_t gb = P + blo;
         uint32_t base = FWD ? (n >> (gb + 1)) : n + 1u - (n >> gb);
         if constexpr (!COL) base += o << j;
-        return ld_g2(tw + base + lp);
+        return ld_g2(tw + base + (kt >> (P + 1)) + pr);
     }
 };
 
@@ -250,9 +253,66 @@ template <int LOG_G, bool FWD>
 struct TwLds {
     const ulonglong2 *t;
     template <int P>
-    __device__ __forceinline__ ulonglong2 get(uint32_t lp) const {
+    __device__ __forceinline__ ulonglong2 get(uint32_t kt, uint32_t pr) const {
         constexpr int j = LOG_G - 1 - P;
-        return t[tw_lds_index<LOG_G, FWD>(j, lp)];
+        return t[tw_lds_index<LOG_G, FWD>(j, (kt >> (P + 1)) + pr)];
+    }
+};
+
+// Transposed LDS layout of a ROW pass's twiddles.  Stage p lies in chunk c
+// (bits [LO, LO+B) in registers).  A chunk narrower than the radix (B <
+// LOG_E, only ever the last) also keeps the thread's spare register bits,
+// and Geom::kr places those that do not fit below LO just above the chunk,
+// so the registers cover [LO, TOP) with TOP = LO + B + max(0, LOG_E - B - LO)
+// and the thread's bits lie above TOP.  The stage's entry index
+// lp = index >> (p+1) is then the thread's bits above TOP, th
+// (nth = LOG_G - TOP of them), over the register bits [p+1, TOP)
+// (nreg = TOP - 1 - p of them):
+//     lp = th << nreg | rg.
+// Lanes of a row differ in th, so in the table's own order they read entries
+// 2^nreg apart, which strided 16-byte LDS reads serve with bank conflicts.
+// The copy stores each stage's 2^j block as pos = rg << nth | th instead:
+// for a given register the lanes read consecutive entries (and rows of one
+// wave read the same ones).  Same entries, new positions.
+template <int LOG_G, int LOG_E, bool FWD>
+struct TwLayout {
+    using Gm = Geom<LOG_G, LOG_E, FWD>;
+    static constexpr int chunk_of(int p) {
+        for (int c = 0; c < Gm::NC; ++c)
+            if (p >= Gm::LO(c) && p < Gm::LO(c) + Gm::B(c)) return c;
+        return 0;
+    }
+    static constexpr int top(int c) {
+        const int spare = LOG_E - Gm::B(c) - Gm::LO(c);
+        return Gm::LO(c) + Gm::B(c) + (spare > 0 ? spare : 0);
+    }
+    static constexpr int nth(int p) { return LOG_G - top(chunk_of(p)); }
+    static constexpr int nreg(int p) { return LOG_G - 1 - p - nth(p); }
+    // nth of every stage by j = LOG_G - 1 - p, four bits per j (LOG_G <= 16)
+    static constexpr uint64_t nth_table() {
+        uint64_t t = 0;
+        for (int j = 0; j < LOG_G; ++j) t |= (uint64_t)nth(LOG_G - 1 - j) << (4 * j);
+        return t;
+    }
+    // entry lp of stage j for block position pos (the inverse of the transposition)
+    __device__ static __forceinline__ uint32_t lp_of(uint32_t j, uint32_t pos) {
+        const uint32_t nt = (uint32_t)(nth_table() >> (4 * j)) & 15u;
+        const uint32_t nr = j - nt;
+        return ((pos & ((1u << nt) - 1u)) << nr) | (pos >> nt);
+    }
+};
+
+template <int LOG_G, int LOG_E, bool FWD>
+struct TwLdsT {
+    using Lay = TwLayout<LOG_G, LOG_E, FWD>;
+    const ulonglong2 *t;
+    template <int P>
+    __device__ __forceinline__ ulonglong2 get(uint32_t kt, uint32_t pr) const {
+        constexpr int j = LOG_G - 1 - P;
+        constexpr int nt = Lay::nth(P), nr = Lay::nreg(P);
+        static_assert(nt + nr == j, "stage block size");
+        // th = kt >> (P + 1 + nr) = kt >> TOP: the same shift for every stage of the chunk
+        return t[tw_lds_index<LOG_G, FWD>(j, (pr << nt) | (kt >> (P + 1 + nr)))];
     }
 };
 
@@ -277,7 +337,11 @@ struct TwStage {
     //     inverse:  (o + 2 - 2R) << j  +  i + n + 1 - G        j = floor(log2(G - 1 - i))
     // (the inverse's uniform term is negative for most rows; the sum is
     // formed modulo 2^32 and lands in [0, n)).
-    template <bool FWD>
+    // LOG_E >= 0 stages the copy in the transposed layout of TwLdsT for a
+    // ROW pass with 2^LOG_E words per thread: slot i, position pos of its
+    // stage's block, holds entry lp_of(j, pos), i.e. i + (lp - pos) in the
+    // table order above.
+    template <bool FWD, int LOG_E = -1>
     __device__ __forceinline__ void fetch_row(const ulonglong2 *tw, uint32_t n, uint32_t o, uint32_t t) {
         const uint32_t rows = n >> LOG_G;
         const uint32_t c = FWD ? rows + o - 1u : o + 2u - 2u * rows;  // uniform
@@ -286,7 +350,12 @@ struct TwStage {
             constexpr int k = decltype(kc)::value;
             const uint32_t i = min(t + (uint32_t)k * NT, G - 2u);
             const uint32_t j = 31u - __clz(FWD ? i + 1u : G - 1u - i);
-            v[k] = ld_g2(tw + ((c << j) + i + d));
+            uint32_t e = i;
+            if constexpr (LOG_E >= 0) {
+                const uint32_t pos = FWD ? i + 1u - (1u << j) : i - (G - (2u << j));
+                e = i - pos + TwLayout<LOG_G, LOG_E, FWD>::lp_of(j, pos);
+            }
+            v[k] = ld_g2(tw + ((c << j) + e + d));
         });
     }
 
@@ -364,8 +433,6 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
             constexpr int s = decltype(sc)::value;
             constexpr int p = FWD ? LO + B - 1 - s : LO + s;
             constexpr int bit = 1 << (p - LO);
-            // thread part of the entry offset lp; the register part is added per pair
-            const uint32_t pt = ktid >> (p + 1);
             const bool fin = !FWD && final_inv && (p == LOG_G - 1);
             uint64_t w[E], wq[E];
             static_for<0, E>([&](auto rc) {
@@ -375,7 +442,7 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
                     constexpr uint32_t pr = Gm::kr(c, r) >> (p + 1);
                     if constexpr (rp == r) {
                         if (!fin) {
-                            const ulonglong2 tv = tws.template get<p>(pt + pr);
+                            const ulonglong2 tv = tws.template get<p>(ktid, pr);
                             w[r] = tv.x;
                             wq[r] = tv.y;
                         }
@@ -642,7 +709,7 @@ ntt_row_stack_kernel(PassArgs a) {
 
     ulonglong2 *tw_lds = reinterpret_cast<ulonglong2 *>(lds + L::WORDS);
     TwStage<LOG_G, NT> tstage;
-    tstage.template fetch_row<FWD>(J0.tw, n, o, t);
+    tstage.template fetch_row<FWD, LOG_E>(J0.tw, n, o, t);
 
     const uint64_t row = (uint64_t)o << LOG_G;
     uint64_t v[E];
@@ -686,7 +753,7 @@ ntt_row_stack_kernel(PassArgs a) {
         __syncthreads();  // publishes the twiddles
     }
 
-    const TwLds<LOG_G, FWD> tws{tw_lds};
+    const TwLdsT<LOG_G, LOG_E, FWD> tws{tw_lds};
     run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX>(v, lds, g, tid, m, tws, false, J0);
 
     if (a.normalize) {

@@ -191,8 +191,8 @@ polymul_row_stack_kernel(PolyArgs a) {
     ulonglong2 *twf_lds = reinterpret_cast<ulonglong2 *>(lds + L::WORDS);
     ulonglong2 *twi_lds = twf_lds + (G - 1);
     TwStage<LOG_G, NT> sf, si;
-    sf.template fetch_row<true>(P0.twf, n, o, t);
-    si.template fetch_row<false>(P0.twi, n, o, t);
+    sf.template fetch_row<true, LOG_E>(P0.twf, n, o, t);
+    si.template fetch_row<false, LOG_E>(P0.twi, n, o, t);
 
     NttJobDev fin{};  // no final stage here: it belongs to the inverse COL pass
 
@@ -205,8 +205,8 @@ polymul_row_stack_kernel(PolyArgs a) {
     sf.store(twf_lds, t);
     si.store(twi_lds, t);
     __syncthreads();  // publishes the twiddles
-    const TwLds<LOG_G, true> twf{twf_lds};
-    const TwLds<LOG_G, false> twi{twi_lds};
+    const TwLdsT<LOG_G, LOG_E, true> twf{twf_lds};
+    const TwLdsT<LOG_G, LOG_E, false> twi{twi_lds};
     run_chunks<LOG_G, LOG_E, GPW, true, false, RLX>(va, lds, g, tid, m, twf, false, fin);
     static_for<0, E>([&](auto rc) {
         constexpr int r = decltype(rc)::value;
