@@ -143,6 +143,7 @@ SIGNATURES = {
     "axhel_hip_ntt_batch_create": (_st, [ctypes.POINTER(NttJob), ctypes.c_size_t, ctypes.POINTER(_p)]),
     "axhel_hip_ntt_batch_launch": (_st, [_p, _p]),
     "axhel_hip_ntt_batch_launch_eager": (_st, [_p, _p]),
+    "axhel_hip_ntt_batch_launch_graph": (_st, [_p, _p]),
     "axhel_hip_ntt_batch_destroy": (_st, [_p]),
     "axhel_hip_ntt_batch_kernel_count": (ctypes.c_int, [_p]),
     "axhel_hip_ntt_batch_launch_kernel": (_st, [_p, ctypes.c_int, _p]),
@@ -380,7 +381,11 @@ class Batch:
         self._h = h
 
     def launch(self, stream=None):
+        """One (degree, direction) group: direct kernel launches; several: the graph."""
         _check(_load().axhel_hip_ntt_batch_launch(self._h, _stream_ptr(stream)), "batch_launch")
+
+    def launch_graph(self, stream=None):
+        _check(_load().axhel_hip_ntt_batch_launch_graph(self._h, _stream_ptr(stream)), "batch_launch_graph")
 
     def launch_eager(self, stream=None):
         _check(_load().axhel_hip_ntt_batch_launch_eager(self._h, _stream_ptr(stream)), "batch_launch_eager")

@@ -217,7 +217,21 @@ axhel_hip_status axhel_hip_ntt_batch_create(const axhel_hip_ntt_job *jobs, size_
     return AXHEL_HIP_OK;
 }
 
+// A batch of one group is a plain chain of its passes: the graph gives it no
+// parallel branches, and back-to-back replays of one graph ran 1.5-2.5 %
+// slower than the same kernels launched directly (0.629 against 0.614 ms for
+// 64 x 24 x 2^16), so launch issues them directly.  Several groups replay
+// the graph, whose branches run the groups side by side.
 axhel_hip_status axhel_hip_ntt_batch_launch(axhel_hip_batch *batch, void *stream) {
+    if (!batch) return AXHEL_HIP_INVALID_ARGUMENT;
+    if (batch->groups.size() == 1) {
+        AXHEL_HIP_TRY(issue(batch, as_stream(stream)));
+        return AXHEL_HIP_OK;
+    }
+    return axhel_hip_ntt_batch_launch_graph(batch, stream);
+}
+
+axhel_hip_status axhel_hip_ntt_batch_launch_graph(axhel_hip_batch *batch, void *stream) {
     if (!batch) return AXHEL_HIP_INVALID_ARGUMENT;
     if (!batch->exec) return AXHEL_HIP_OK;  // empty batch
     AXHEL_HIP_TRY(hipGraphLaunch(batch->exec, as_stream(stream)));

@@ -144,8 +144,13 @@ typedef struct axhel_hip_batch axhel_hip_batch;
    captured into one hipGraph.  Every job pointer must be device memory and no
    two jobs may write overlapping memory. */
 axhel_hip_status axhel_hip_ntt_batch_create(const axhel_hip_ntt_job *jobs, size_t count, axhel_hip_batch **batch);
-/* Replays the captured graph on `stream` (asynchronous). */
+/* Runs the batch on `stream` (asynchronous): a batch of one (degree,
+   direction) group as plain kernel launches of its passes, any other as a
+   replay of the captured graph, whose parallel branches run the groups side
+   by side. */
 axhel_hip_status axhel_hip_ntt_batch_launch(axhel_hip_batch *batch, void *stream);
+/* Replays the captured graph on `stream`, whatever the group count. */
+axhel_hip_status axhel_hip_ntt_batch_launch_graph(axhel_hip_batch *batch, void *stream);
 /* Same work as launch, issued as plain kernel launches without the graph. */
 axhel_hip_status axhel_hip_ntt_batch_launch_eager(axhel_hip_batch *batch, void *stream);
 axhel_hip_status axhel_hip_ntt_batch_destroy(axhel_hip_batch *batch);

@@ -82,9 +82,46 @@ def test_uneven_batch_graph_replays():
         x = torch.arange(1 << 20, device="cuda").sum()  # unrelated eager work between replays
         torch.cuda.synchronize()
         assert int(x) == (1 << 20) * ((1 << 20) - 1) // 2
-    b.launch_eager()
-    for i, (o, want) in enumerate(refs):
-        assert np.array_equal(host(o), want), ("eager", i)
+    for mode in ("eager", "graph"):
+        for o, _ in refs:
+            o.fill_(-1)
+        b.launch_eager() if mode == "eager" else b.launch_graph()
+        for i, (o, want) in enumerate(refs):
+            assert np.array_equal(host(o), want), (mode, i)
+    b.close()
+
+
+def test_single_group_batch_every_launch_mode():
+    """A batch of one (degree, direction) group: launch issues its passes
+    directly, launch_graph replays the captured graph, launch_eager issues
+    them as plain launches; each gives the oracle's output on every replay,
+    with unrelated eager work and synchronizations between replays."""
+    torch = torch_mod()
+    n = 1 << 14
+    qs = params.rns_primes(n, 6)
+    rng = np.random.default_rng(11)
+    jobs, refs = [], []
+    for q in qs:
+        psi = params.find_primitive_2n_root(n, q)
+        plan, tab = ax.NTT(n, q, psi), O.Tables(n, q, psi)
+        for _ in range(3):
+            a = params.uniform(rng, n, 4 * q)
+            s, o = dev(a), empty_dev(n)
+            jobs.append((plan, o, s, False, 1))
+            refs.append((o, s, O.compute_forward(a, tab, 4, 1)))
+    b = ax.Batch(jobs)
+    assert b.kernel_count() >= 1
+    for rep in range(3):
+        for mode in ("launch", "graph", "eager"):
+            for o, _, _ in refs:
+                o.fill_(-1)
+            {"launch": b.launch, "graph": b.launch_graph, "eager": b.launch_eager}[mode]()
+            torch.cuda.synchronize()
+            for i, (o, _, want) in enumerate(refs):
+                assert np.array_equal(host(o), want), (rep, mode, i)
+            x = torch.arange(1 << 18, device="cuda").sum()
+            torch.cuda.synchronize()
+            assert int(x) == (1 << 18) * ((1 << 18) - 1) // 2
     b.close()
 
 
@@ -179,6 +216,7 @@ def test_empty_batch_and_empty_product():
     b = ax.Batch([])
     assert b.kernel_count() == 0 and b.stacked_groups() == 0
     b.launch()
+    b.launch_graph()
     b.launch_eager()
     torch.cuda.synchronize()
     b.close()
