@@ -94,7 +94,7 @@ def tune_single(logn, torch, calls=200):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--logn", type=int, nargs="*", default=list(range(10, 18)))
+    ap.add_argument("--logn", type=int, nargs="*", default=list(range(10, 21)))
     ap.add_argument("--out", default=DEFAULT_OUT)
     ap.add_argument("--words", type=int, default=64 * 24 << 16,
                     help="coefficients per timed batch (default: the 64 x 24 x 2^16 RNS batch)")
@@ -109,18 +109,23 @@ def main():
         ax.EltwiseReduceMod(x, x, x.numel(), (1 << 50) - 27, 4, 2)
         torch.cuda.synchronize()
     del x
-    table = {}
+    header = "# log2(N) batch-config-index single-call-config-index, chosen by axhel_amd.autotune on gfx950\n"
+    table, notes = {}, []
     if os.path.exists(args.out):
         for line in open(args.out):
             parts = line.split()
-            if len(parts) in (2, 3) and not line.startswith("#"):
+            if line.startswith("#"):
+                if line != header:
+                    notes.append(line)  # hand-written notes and overrides are kept
+            elif len(parts) in (2, 3):
                 table[int(parts[0])] = [int(x) for x in parts[1:]]
     for logn in args.logn:
         best, _ = tune_degree(logn, torch, args.words, args.reps)
         single, _ = tune_single(logn, torch)
         table[logn] = [best, single]
     with open(args.out, "w") as f:
-        f.write("# log2(N) batch-config-index single-call-config-index, chosen by axhel_amd.autotune on gfx950\n")
+        f.write(header)
+        f.writelines(notes)
         for logn in sorted(table):
             f.write(f"{logn} {' '.join(map(str, table[logn]))}\n")
     print("wrote", args.out, table, file=sys.stderr)

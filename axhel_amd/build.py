@@ -26,7 +26,8 @@ OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["ntt_cfg_16a.hip", "ntt_cfg_16b.hip", "ntt_cfg_14_15.hip", "ntt_cfg_17.hip", "ntt_cfg_1_13.hip",
+HIP_SOURCES = ["ntt_cfg_16a.hip", "ntt_cfg_16b.hip", "ntt_cfg_14_15.hip", "ntt_cfg_17.hip", "ntt_cfg_18_19.hip", "ntt_cfg_20.hip",
+               "ntt_cfg_1_13.hip",
                "common.hip", "eltwise.hip", "ntt.hip", "batch.hip", "serve.hip", "ubench.hip", "polymul.hip", "rns.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

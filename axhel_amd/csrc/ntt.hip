@@ -111,6 +111,8 @@ std::vector<std::vector<NttConfig>> build_registry() {
     add_configs_16a(r);
     add_configs_16b(r);
     add_configs_17(r);
+    add_configs_18_19(r);
+    add_configs_20(r);
     return r;
 }
 

@@ -24,7 +24,7 @@ struct axhel_hip_plan {
 
 namespace axhel_host {
 
-constexpr uint32_t kMaxLogN = 17;
+constexpr uint32_t kMaxLogN = 20;
 
 // One compiled pass kernel.
 struct PassKernel {

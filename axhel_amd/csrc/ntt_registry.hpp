@@ -121,5 +121,7 @@ void add_configs_14_15(Registry &r);
 void add_configs_16a(Registry &r);
 void add_configs_16b(Registry &r);
 void add_configs_17(Registry &r);
+void add_configs_18_19(Registry &r);
+void add_configs_20(Registry &r);
 
 }  // namespace axhel_host

@@ -91,7 +91,7 @@ typedef struct axhel_hip_plan axhel_hip_plan;
 
 /* NTT::NTT(degree, modulus, root_of_unity): include/axhel/ntt/ntt.hpp:32
    (ntt.cpp:136-152, tables ntt-tables.cpp:81-140).  degree must be a power of
-   two in [2, 2^17] with modulus == 1 mod 2*degree, modulus < 2^62. */
+   two in [2, 2^20] with modulus == 1 mod 2*degree, modulus < 2^62. */
 axhel_hip_status axhel_hip_ntt_plan_create(uint64_t degree, uint64_t modulus, uint64_t root_of_unity,
                                            axhel_hip_plan **plan);
 /* NTT::NTT(degree, modulus, root_powers, inv_root_powers, inv_degree_modulo):

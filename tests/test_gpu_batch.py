@@ -145,7 +145,7 @@ def test_in_place_batch_jobs():
         assert np.array_equal(host(d), want)
 
 
-@pytest.mark.parametrize("logn", [12, 13, 14, 15, 16, 17])
+@pytest.mark.parametrize("logn", [12, 13, 14, 15, 16, 17, 18, 19, 20])
 def test_stacked_row_pass_every_config(logn):
     """Many jobs per plan: the ROW pass runs over stacks of same-plan jobs
     (one twiddle set per workgroup, a padded last stack per plan).  For every

@@ -76,7 +76,7 @@ def _check_ntt(ntt, t, n, q, rng, label):
         assert np.array_equal(host(r), O.compute_forward(a, t, 4, 1)), ("fwd norm", label)
 
 
-@pytest.mark.parametrize("logn", [1, 2, 4, 8, 10, 12, 13, 14, 15, 16, 17])
+@pytest.mark.parametrize("logn", [1, 2, 4, 8, 10, 12, 13, 14, 15, 16, 17, 18, 19, 20])
 def test_ntt_out_of_contract_words_every_config(logn):
     n = 1 << logn
     rng = np.random.default_rng(SEED ^ logn)

@@ -48,7 +48,7 @@ def run_case(n, limbs, fused, with_addend, seed):
     pm.close()
 
 
-@pytest.mark.parametrize("logn", [10, 12, 13, 14, 15, 16, 17])
+@pytest.mark.parametrize("logn", [10, 12, 13, 14, 15, 16, 17, 18, 20])
 @pytest.mark.parametrize("fused", [True, False])
 def test_polymul_matches_call_sequence(logn, fused):
     run_case(1 << logn, 3, fused, True, logn)

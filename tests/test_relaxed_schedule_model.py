@@ -83,7 +83,7 @@ def splits(n):
     return sorted(out)
 
 
-@pytest.mark.parametrize("logn", [1, 2, 5, 8, 10, 12, 13, 14, 15, 16, 17])
+@pytest.mark.parametrize("logn", [1, 2, 5, 8, 10, 12, 13, 14, 15, 16, 17, 18, 19, 20])
 def test_relaxed_schedule_bound_and_result(logn):
     n = 1 << logn
     q = params.find_ntt_prime(n, 60)  # largest NTT prime below the 2^60 gate
