@@ -308,8 +308,11 @@ class NTT:
         if root_of_unity is not None:
             s = L.axhel_hip_ntt_plan_create(degree, modulus, root_of_unity, ctypes.byref(h))
         else:
+            # borrowed for the plan's life (ntt.hpp:34-37): a uint64 contiguous
+            # array is used in place, so later changes to it reach the calls
             rp = np.ascontiguousarray(root_powers, dtype=np.uint64)
             ip = np.ascontiguousarray(inv_root_powers, dtype=np.uint64)
+            self._tables = (rp, ip)
             s = L.axhel_hip_ntt_plan_create_external(degree, modulus, rp.ctypes.data, ip.ctypes.data,
                                                      int(inv_degree_modulo[0]), int(inv_degree_modulo[1]),
                                                      ctypes.byref(h))

@@ -350,16 +350,21 @@ using namespace axhel_host;
 
 namespace {
 
-// Finishes a plan whose ninv, ninv_q are set: the final inverse stage's
-// scaled root from wl = inv_root_powers[N-1].operand (ntt-native.cpp:109-113,
-// computed once per plan instead of once per call; same value), then the
-// device tables: uploaded from h_fwd/h_inv when the caller supplied them
-// (external tables), else generated on the device from psi.
-axhel_hip_status finish_plan(axhel_hip_plan *p, uint64_t wl, uint64_t psi, uint64_t ipsi, axhel_hip_plan **out) {
+// The scaled root of the final inverse stage from wl = inv_root_powers[N-1]
+// (ntt-native.cpp:109-113, once per table instead of once per call).
+void set_final_root(axhel_hip_plan *p, uint64_t wl) {
     uint64_t s = wl * p->ninv - (uint64_t)(((u128)wl * p->ninv_q) >> 64) * p->q;
     if (s >= p->q) s -= p->q;
     p->sroot = s;
     p->sroot_q = shoup_quotient(s, p->q);
+}
+
+// Finishes a plan whose ninv, ninv_q are set: the final inverse stage's
+// scaled root from wl = inv_root_powers[N-1].operand, then the device
+// tables: uploaded from h_fwd/h_inv when the caller supplied them (external
+// tables), else generated on the device from psi.
+axhel_hip_status finish_plan(axhel_hip_plan *p, uint64_t wl, uint64_t psi, uint64_t ipsi, axhel_hip_plan **out) {
+    set_final_root(p, wl);
     hipError_t e = hipGetDevice(&p->device);
     if (e == hipSuccess) e = hipMalloc(&p->d_fwd, p->n * sizeof(ulonglong2));
     if (e == hipSuccess) e = hipMalloc(&p->d_inv, p->n * sizeof(ulonglong2));
@@ -396,9 +401,30 @@ bool valid_degree(uint64_t degree, uint32_t *logn) {
     return true;
 }
 
+// A plan on borrowed tables reads them as they are at each call, as the
+// reference does: when the table this call uses differs from what the
+// device holds, every transform issued so far is let finish (they read the
+// old one) and the table is reloaded.  The comparison reads the 16 N bytes
+// that the reference's own transform reads from the table on every call.
+hipError_t reload_external(const axhel_hip_plan *cp, bool inverse) {
+    axhel_hip_plan *p = const_cast<axhel_hip_plan *>(cp);
+    std::lock_guard<std::mutex> lk(p->ext_mu);
+    const uint64_t *src = inverse ? p->ext_inv : p->ext_fwd;
+    std::vector<uint64_t> &held = inverse ? p->h_inv : p->h_fwd;
+    const size_t bytes = 2 * p->n * sizeof(uint64_t);
+    if (std::memcmp(src, held.data(), bytes) == 0) return hipSuccess;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return e;
+    std::memcpy(held.data(), src, bytes);
+    e = hipMemcpy(inverse ? p->d_inv : p->d_fwd, held.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess && inverse) set_final_root(p, held[2 * (p->n - 1)]);
+    return e;
+}
+
 axhel_hip_status run_one(const axhel_hip_plan *plan, uint64_t *result, const uint64_t *operand, bool inverse,
                          bool normalize, void *stream) {
     if (!plan || !result || !operand) return AXHEL_HIP_INVALID_ARGUMENT;
+    if (plan->ext_fwd) AXHEL_HIP_TRY(reload_external(plan, inverse));
     const hipStream_t st = as_stream(stream);
     Staging stg(st);
     const uint64_t *dop = nullptr;
@@ -456,6 +482,8 @@ axhel_hip_status axhel_hip_ntt_plan_create_external(uint64_t degree, uint64_t mo
     p->q = modulus;
     p->h_fwd.assign(root_powers, root_powers + 2 * degree);
     p->h_inv.assign(inv_root_powers, inv_root_powers + 2 * degree);
+    p->ext_fwd = root_powers;
+    p->ext_inv = inv_root_powers;
     p->ninv = inv_degree_operand;
     p->ninv_q = inv_degree_quotient;
     return finish_plan(p, inv_root_powers[2 * (degree - 1)], 0, 0, plan);

@@ -97,7 +97,13 @@ axhel_hip_status axhel_hip_ntt_plan_create(uint64_t degree, uint64_t modulus, ui
 /* NTT::NTT(degree, modulus, root_powers, inv_root_powers, inv_degree_modulo):
    ntt.hpp:37 (ntt.cpp:124-133).  Tables are host arrays of `degree`
    {operand, quotient} pairs (2*degree words, the NTTMultiplyOperand layout of
-   ntt.hpp:14-18); they are copied to the device once. */
+   ntt.hpp:14-18), borrowed for the plan's life as the reference borrows them:
+   they must stay valid until the plan is destroyed.  The device holds a copy;
+   axhel_hip_ntt_forward / _inverse compare the table they use with it and,
+   if the caller has changed it, wait for the device to finish the work
+   issued so far and reload it, so a call sees the tables as they are when it
+   is made, as in the reference.  Batches, RNS contexts, products and the
+   server use the device copy as last loaded. */
 axhel_hip_status axhel_hip_ntt_plan_create_external(uint64_t degree, uint64_t modulus, const uint64_t *root_powers,
                                                     const uint64_t *inv_root_powers, uint64_t inv_degree_operand,
                                                     uint64_t inv_degree_quotient, axhel_hip_plan **plan);

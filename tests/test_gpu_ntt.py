@@ -123,6 +123,39 @@ def test_external_tables_constructor():
     assert np.array_equal(host(x), O.compute_forward(a, t, 1, 4))
 
 
+def test_external_tables_are_borrowed():
+    """ntt.hpp:34-37: the plan borrows the caller's tables, so a call sees
+    them as they are when it is made.  The caller rewrites its arrays in
+    place with the tables of another primitive 2N-th root (psi^3), and back;
+    each call equals the oracle on the tables it saw, forward and inverse
+    (the inverse's scaled final root follows the last inverse entry)."""
+    n = 2048
+    q, psi = ntt_params(n, 55)
+    t1 = O.Tables(n, q, psi)
+    t3 = O.Tables(n, q, pow(psi, 3, q))
+    fwd = t1.fwd.copy()
+    inv = t1.inv.copy()
+    ext = ax.NTT(n, q, root_powers=fwd, inv_root_powers=inv, inv_degree_modulo=t1.ninv)
+    a4 = O.MT19937_64(5).random_vector(n, 4 * q)
+    a2 = O.MT19937_64(6).random_vector(n, 2 * q)
+    r = empty_dev(n)
+    for t in (t1, t3, t1, t3):
+        fwd[:] = t.fwd
+        inv[:] = t.inv
+        for of in (4, 1):
+            ext.ComputeForward(r, dev(a4), 4, of)
+            assert np.array_equal(host(r), O.compute_forward(a4, t, 4, of)), (t is t3, of)
+        for of in (2, 1):
+            ext.ComputeInverse(r, dev(a2), 2, of)
+            assert np.array_equal(host(r), O.compute_inverse(a2, t, 2, of)), (t is t3, of)
+    # a change to one table only: the forward table back to psi, the inverse kept at psi^3
+    fwd[:] = t1.fwd
+    ext.ComputeForward(r, dev(a4), 4, 1)
+    assert np.array_equal(host(r), O.compute_forward(a4, t1, 4, 1))
+    ext.ComputeInverse(r, dev(a2), 2, 1)
+    assert np.array_equal(host(r), O.compute_inverse(a2, t3, 2, 1))
+
+
 def test_special_patterns():
     """test_ntt.cpp:15-30 patterns, lazy and normalized, against the oracle."""
     for n in (8, 4096, 65536):
