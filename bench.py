@@ -317,23 +317,26 @@ def steady_warmup(torch, fn, min_steps, block=5, tol=0.02, max_seconds=3.0):
             f"within {tol:.0%} (at most {max_seconds:.0f} s)", "last_block_ms_per_step": last}
 
 
-def kernel_durations(torch, batch, reps, stream):
-    """Average duration of each launch of a replay: reps launches of the
-    kernel back to back between one pair of HIP events on the stream it runs
-    on (the host's launch latency hides behind the running kernels)."""
-    out = []
-    with torch.cuda.stream(stream):
-        for k in range(batch.kernel_count()):
-            for _ in range(3):
-                batch.launch_kernel(k, stream)  # warm
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            for _ in range(reps):
-                batch.launch_kernel(k, stream)
-            b.record(stream)
-            stream.synchronize()
-            out.append(a.elapsed_time(b) / reps)
-    return out
+def instrumented_steps(torch, batch, steps, stream):
+    """The timed steps once more, issued pass by pass with a HIP event
+    recorded on the kernels' stream before each pass kernel and after the
+    last: each launch's duration as it runs inside the step sequence (its
+    neighbours the other passes, as in the timed region), averaged over the
+    steps, and the wall time per step of this copy (to show the events do
+    not change the step)."""
+    nk = batch.kernel_count()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(nk + 1)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        for k in range(nk):
+            evs[s][k].record(stream)
+            batch.launch_kernel(k, stream)
+        evs[s][nk].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    per = [float(np.mean([evs[s][k].elapsed_time(evs[s][k + 1]) for s in range(steps)])) for k in range(nk)]
+    return per, wall
 
 
 def schedule_peaks(ax, seconds=0.3):
@@ -386,14 +389,15 @@ def mix_roofline(infos, ms, peaks):
             "launches": len(infos)}
 
 
-def batch_leg(ax, torch, batch, limbs, steps, warmup, stream, peaks, reps):
+def batch_leg(ax, torch, batch, limbs, steps, warmup, stream, peaks):
     """A batch timed as Batch.launch calls on `stream` after a steady warm-up,
-    with each launch's roofline."""
+    with each launch's roofline from the instrumented copy of the steps."""
     wu = steady_warmup(torch, lambda: batch.launch(stream), warmup)
     ms = sync_time(torch, lambda: batch.launch(stream), steps)
-    kd = kernel_durations(torch, batch, reps, stream)
+    kd, iwall = instrumented_steps(torch, batch, steps, stream)
     kern = [kernel_roofline(batch.kernel_info(k), kd[k], peaks) for k in range(len(kd))]
-    return {"ms_per_step": ms, "limbs_per_s": limbs / (ms * 1e-3), "warmup": wu, "kernels": kern}
+    return {"ms_per_step": ms, "limbs_per_s": limbs / (ms * 1e-3), "warmup": wu, "kernels": kern,
+            "instrumented_ms_per_step": iwall}
 
 
 def main():
@@ -405,7 +409,6 @@ def main():
     ap.add_argument("--limbs", type=int, default=24)
     ap.add_argument("--logn", type=int, default=16)
     ap.add_argument("--plain-limbs", type=int, default=384, help="limbs timed on the plain per-call path")
-    ap.add_argument("--kernel-reps", type=int, default=20, help="back-to-back launches per kernel duration")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eltwise", action="store_true")
@@ -494,12 +497,12 @@ def main():
     value = limbs * args.steps / el
     log(f"batched forward (Batch.launch, default stream): {ms:.3f} ms/step, {value:.4g} limbs/s")
 
-    # per-kernel durations with HIP events on the stream the kernels run on,
-    # and the register-resident peak of each pass schedule
-    stream = torch.cuda.Stream()
-    kdur = kernel_durations(torch, batch, args.kernel_reps, stream)
+    # per-kernel durations: the same steps again on the default stream with a
+    # HIP event between the pass kernels; then the register-resident peak of
+    # each pass schedule
+    kdur, instr_ms = instrumented_steps(torch, batch, args.steps, torch.cuda.default_stream())
     peaks = schedule_peaks(ax)
-    log("kernel avg ms:", [round(x, 4) for x in kdur])
+    log(f"kernel avg ms: {[round(x, 4) for x in kdur]} (instrumented copy of the steps: {instr_ms:.3f} ms/step)")
     log("schedule peaks Gbfly/s:", {v["schedule"]: round(v["peak"], 1) for v in peaks.values()})
     kern = [kernel_roofline(batch.kernel_info(k), kdur[k], peaks) for k in range(len(kdur))]
     dom = int(np.argmax(kdur))
@@ -524,7 +527,7 @@ def main():
         leg_specs += [("lazy_forward", aux, src, False, 4), ("lazy_inverse", aux, dst, True, 2)]
     for name, out_t, in_t, inverse, factor in leg_specs:
         b = ax.Batch([(plans[l], out_t[p, l], in_t[p, l], inverse, factor) for p in range(P) for l in range(L)])
-        legs[name] = batch_leg(ax, torch, b, limbs, args.steps, args.warmup, run_stream, peaks, args.kernel_reps)
+        legs[name] = batch_leg(ax, torch, b, limbs, args.steps, args.warmup, run_stream, peaks)
         b.close()
         lk = legs[name]["kernels"]
         ld = int(np.argmax([k["ms"] for k in lk]))
@@ -624,6 +627,10 @@ def main():
         "lazy_forward_limbs_per_s": legs.get("lazy_forward", {}).get("limbs_per_s"),
         "lazy_inverse_limbs_per_s": legs.get("lazy_inverse", {}).get("limbs_per_s"),
         "kernel_ms": kdur,
+        "kernel_timing": "HIP events on the default stream between the pass kernels of an instrumented copy of "
+                         "the timed steps (the same batch and step count, issued pass by pass right after the "
+                         "timed region), ms per launch averaged over the steps",
+        "instrumented_ms_per_step": instr_ms,
         "kernels": kern,
         "legs": legs,
         "schedule_peaks": peaks,
