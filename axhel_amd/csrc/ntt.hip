@@ -406,9 +406,10 @@ bool valid_degree(uint64_t degree, uint32_t *logn) {
 // device holds, every transform issued so far is let finish (they read the
 // old one) and the table is reloaded.  The comparison reads the 16 N bytes
 // that the reference's own transform reads from the table on every call.
+// Called with the plan's ext_mu held; the plan is shared as const, and its
+// device copy is the one part a call may rewrite.
 hipError_t reload_external(const axhel_hip_plan *cp, bool inverse) {
     axhel_hip_plan *p = const_cast<axhel_hip_plan *>(cp);
-    std::lock_guard<std::mutex> lk(p->ext_mu);
     const uint64_t *src = inverse ? p->ext_inv : p->ext_fwd;
     std::vector<uint64_t> &held = inverse ? p->h_inv : p->h_fwd;
     const size_t bytes = 2 * p->n * sizeof(uint64_t);
@@ -424,17 +425,25 @@ hipError_t reload_external(const axhel_hip_plan *cp, bool inverse) {
 axhel_hip_status run_one(const axhel_hip_plan *plan, uint64_t *result, const uint64_t *operand, bool inverse,
                          bool normalize, void *stream) {
     if (!plan || !result || !operand) return AXHEL_HIP_INVALID_ARGUMENT;
-    if (plan->ext_fwd) AXHEL_HIP_TRY(reload_external(plan, inverse));
     const hipStream_t st = as_stream(stream);
     Staging stg(st);
     const uint64_t *dop = nullptr;
     uint64_t *dres = nullptr;
     AXHEL_HIP_TRY(stg.input(operand, plan->n, &dop));
     AXHEL_HIP_TRY(stg.output(result, plan->n, &dres));
+    // A plan on borrowed tables holds its lock from the table check until its
+    // kernels are queued, so a reload (which first waits for the device)
+    // never rewrites a table that a queued transform has still to read.
+    std::unique_lock<std::mutex> lk;
+    if (plan->ext_fwd) {
+        lk = std::unique_lock<std::mutex>(plan->ext_mu);
+        AXHEL_HIP_TRY(reload_external(plan, inverse));
+    }
     const NttJobDev job = make_job(plan, dop, dres, inverse);
     const bool vec16 = aligned16(dop) && aligned16(dres);
     AXHEL_HIP_TRY(launch_ntt(config_for_single(plan->logn), plan->logn, inverse, normalize, nullptr, job, 1, st, -1,
                              vec16, nullptr, relax_ok_modulus(plan->q)));
+    if (lk.owns_lock()) lk.unlock();
     AXHEL_HIP_TRY(stg.finish());
     return AXHEL_HIP_OK;
 }

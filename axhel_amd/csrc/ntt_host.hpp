@@ -20,9 +20,10 @@ struct axhel_hip_plan {
     // External tables: the caller's arrays, borrowed for the plan's life as
     // the reference borrows them (ntt.hpp:34-37).  h_fwd/h_inv hold what the
     // device tables were last loaded from; the per-call transforms reload a
-    // table that the caller has changed since.  ext_mu guards the reload.
+    // table that the caller has changed since.  ext_mu serializes the check,
+    // the reload and the queuing of the transform's kernels.
     const uint64_t *ext_fwd = nullptr, *ext_inv = nullptr;
-    std::mutex ext_mu;
+    mutable std::mutex ext_mu;
     uint64_t ninv = 0, ninv_q = 0;
     uint64_t sroot = 0, sroot_q = 0;  // CS(Shoup(inv[n-1], n^-1), q) and its quotient
     ulonglong2 *d_fwd = nullptr;
