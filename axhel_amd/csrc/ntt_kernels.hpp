@@ -396,11 +396,39 @@ __device__ __forceinline__ constexpr FwdStage stage_kind(int p) {
     return s % 4 == 0 ? FwdStage::SUB8 : FwdStage::SKIP;
 }
 
+// Orders one wave's LDS accesses for an exchange among its own lanes.  A
+// wave's LDS instructions execute in issue order, so no hardware barrier is
+// needed; the fences keep the compiler from moving a lane's LDS accesses
+// across this point (it reasons per lane and would see no dependence).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Synchronization of a group's LDS exchange: the workgroup barrier, or
+// wave_lds_sync when every group lies inside one wave and owns its LDS rows
+// (WAVE).
+template <bool WAVE>
+__device__ __forceinline__ void group_sync() {
+    if constexpr (WAVE) wave_lds_sync();
+    else __syncthreads();
+}
+
+// A ROW group of T = 2^(LOG_G - LOG_E) threads lies inside one wave64 when T
+// divides 64; its LDS words (Lds::idx2(g, ...)) are its own.
+template <int LOG_G, int LOG_E>
+constexpr bool row_group_in_wave() {
+    return (64 % (1 << (LOG_G - LOG_E))) == 0;
+}
+
 // The butterfly chunks of one pass over registers v (chunk-0 layout on entry,
 // last-chunk layout on exit), redistributing through LDS between chunks.
 // Every thread of the workgroup must call it (it holds barriers).
 // RLX: forward butterflies on the relaxed schedule (stage_kind).
-template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, bool RLX = false, typename TW>
+// WSYNC: the exchanges synchronize each wave alone (ROW groups inside one
+// wave, row_group_in_wave, whose LDS rows no other wave touches meanwhile).
+template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, bool RLX = false, bool WSYNC = false, typename TW>
 __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *lds, uint32_t g, uint32_t tid,
                                            const ModConsts &m, const TW &tws, bool final_inv, const NttJobDev &J) {
     using Gm = Geom<LOG_G, LOG_E, FWD>;
@@ -415,13 +443,14 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
 
         if constexpr (c > 0) {
             // redistribute: registers in chunk c-1 layout -> LDS -> chunk c layout
+            static_assert(!WSYNC || (!COL && row_group_in_wave<LOG_G, LOG_E>()), "wave-local exchange");
             const uint32_t kp = Gm::kt(c - 1, tid);
-            __syncthreads();
+            group_sync<WSYNC>();
             static_for<0, E>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
                 lds[L::idx2(g, kp, Gm::kr(c - 1, r))] = v[r];
             });
-            __syncthreads();
+            group_sync<WSYNC>();
             static_for<0, E>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
                 v[r] = lds[L::idx2(g, ktid, Gm::kr(c, r))];
@@ -600,8 +629,12 @@ ntt_pass_kernel(PassArgs a) {
         const TwLds<LOG_G, FWD> tws{tw_lds};
         run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
     } else {
+        // a ROW group's exchanges touch its own LDS words only (the staged
+        // loads above and the staged stores below are published by
+        // workgroup barriers): a group inside one wave synchronizes alone
         const TwGlobal<LOG_G, FWD, COL> tws{tw, n, blo, o};
-        run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
+        run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX, row_group_in_wave<LOG_G, LOG_E>()>(v, lds, g, tid, m, tws,
+                                                                                      a.final_inv != 0, J);
     }
 
     // ---- normalisation folded into the last pass (ntt.cpp:79-118)
@@ -753,8 +786,11 @@ ntt_row_stack_kernel(PassArgs a) {
         __syncthreads();  // publishes the twiddles
     }
 
+    // a row's LDS words are its own: when a row lies inside one wave, its
+    // exchanges and its store staging synchronize that wave alone
+    constexpr bool WS = row_group_in_wave<LOG_G, LOG_E>();
     const TwLdsT<LOG_G, LOG_E, FWD> tws{tw_lds};
-    run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX>(v, lds, g, tid, m, tws, false, J0);
+    run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX, WS>(v, lds, g, tid, m, tws, false, J0);
 
     if (a.normalize) {
         static_for<0, E>([&](auto rc) {
@@ -767,12 +803,12 @@ ntt_row_stack_kernel(PassArgs a) {
     const uint32_t kl = Gm::kt(NC - 1, tid);
     if constexpr (FWD) {
         // forward: the last chunk is thread-contiguous; store through LDS
-        __syncthreads();
+        group_sync<WS>();
         static_for<0, E>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
             lds[L::idx2(g, kl, Gm::kr(NC - 1, r))] = v[r];
         });
-        __syncthreads();
+        group_sync<WS>();
         if (valid) {
             if (a.vec16) {
 #pragma unroll

@@ -96,13 +96,16 @@ polymul_row_kernel(PolyArgs a) {
     });
     const TwGlobal<LOG_G, true, false> twf{P.twf, n, 0, o};
     const TwGlobal<LOG_G, false, false> twi{P.twi, n, 0, o};
-    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX>(va, lds, g, tid, m, twf, false, fin);
+    // every LDS word a row touches is its own (Lds::idx2(g, ...)): when a row
+    // lies inside one wave, its exchanges synchronize that wave alone
+    constexpr bool WS = row_group_in_wave<LOG_G, LOG_E>();
+    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX, WS>(va, lds, g, tid, m, twf, false, fin);
     static_for<0, E>([&](auto rc) {
         constexpr int r = decltype(rc)::value;
         vb[r] = ld_g(P.b1 + (row | (k0 + Gf::kr(0, r))));
     });
-    __syncthreads();  // the LDS tile is reused by b's pass
-    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX>(vb, lds, g, tid, m, twf, false, fin);
+    group_sync<WS>();  // the LDS tile is reused by b's pass
+    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX, WS>(vb, lds, g, tid, m, twf, false, fin);
 
     // thread holds words row + kl + r, r in [0, E)
     const uint32_t kl = Gf::kt(Gf::NC - 1, tid);
@@ -135,7 +138,7 @@ polymul_row_kernel(PolyArgs a) {
         }
         va[r] = d;
     });
-    run_chunks<LOG_G, LOG_E, GPW, false, false>(va, lds, g, tid, m, twi, a.single != 0, fin);
+    run_chunks<LOG_G, LOG_E, GPW, false, false, false, WS>(va, lds, g, tid, m, twi, a.single != 0, fin);
     if (a.single) {
         static_for<0, E>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
@@ -207,13 +210,16 @@ polymul_row_stack_kernel(PolyArgs a) {
     __syncthreads();  // publishes the twiddles
     const TwLdsT<LOG_G, LOG_E, true> twf{twf_lds};
     const TwLdsT<LOG_G, LOG_E, false> twi{twi_lds};
-    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX>(va, lds, g, tid, m, twf, false, fin);
+    // the staged twiddles are shared (published above); every other LDS word
+    // a row touches is its own, so a row inside one wave synchronizes alone
+    constexpr bool WS = row_group_in_wave<LOG_G, LOG_E>();
+    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX, WS>(va, lds, g, tid, m, twf, false, fin);
     static_for<0, E>([&](auto rc) {
         constexpr int r = decltype(rc)::value;
         vb[r] = valid ? ld_g(b1 + row + k0 + Gf::kr(0, r)) : 0;
     });
-    __syncthreads();  // the LDS tile is reused by b's pass
-    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX>(vb, lds, g, tid, m, twf, false, fin);
+    group_sync<WS>();  // the LDS tile is reused by b's pass
+    run_chunks<LOG_G, LOG_E, GPW, true, false, RLX, WS>(vb, lds, g, tid, m, twf, false, fin);
 
     const uint32_t kl = Gf::kt(Gf::NC - 1, tid);
     uint64_t cv[E];
@@ -245,7 +251,7 @@ polymul_row_stack_kernel(PolyArgs a) {
         }
         va[r] = d;
     });
-    run_chunks<LOG_G, LOG_E, GPW, false, false>(va, lds, g, tid, m, twi, false, fin);
+    run_chunks<LOG_G, LOG_E, GPW, false, false, false, WS>(va, lds, g, tid, m, twi, false, fin);
     if (valid) {
         const uint32_t ke = Gi::kt(Gi::NC - 1, tid);
         static_for<0, E>([&](auto rc) {
