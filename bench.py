@@ -4,9 +4,10 @@ Workload: 64 polynomials x 24 RNS limbs x N = 65536 uint64 coefficients
 (1536 limb-transforms, 805 MB per operand array), 24 distinct primes of
 50..60 bits (== 1 mod 2N), coefficients uniform in [0, q), resident in HBM.
 One step = the forward negacyclic NTT (ComputeForward with default factors,
-normalized output) of every limb, issued as one replay of the batch's
-captured hipGraph.  Plain baseline = ComputeForward per limb through the same
-library, synchronous per call, on device-resident buffers.
+normalized output) of every limb, issued as one Batch.launch (a batch of one
+degree and direction launches its two pass kernels directly; its captured
+hipGraph is timed as a variant).  Plain baseline = ComputeForward per limb
+through the same library, synchronous per call, on device-resident buffers.
 
 Prints one JSON line.  --gpus above 1 is refused: this build measures one GPU.
 """
@@ -339,6 +340,53 @@ def instrumented_steps(torch, batch, steps, stream):
     return per, wall
 
 
+def _smi_sample():
+    """Socket power (W) and the mean gfx clock over the XCDs (MHz) from
+    amd-smi, or None when the tool is absent or fails."""
+    import subprocess
+    try:
+        r = subprocess.run(["amd-smi", "metric", "-p", "-c", "-g", "0", "--json"], capture_output=True, text=True,
+                           timeout=5)
+        d = json.loads(r.stdout)
+        g = d["gpu_data"][0] if isinstance(d, dict) else d[0]
+        watts = float(g["power"]["socket_power"]["value"])
+        clks = [float(v["clk"]["value"]) for k, v in g["clock"].items() if k.startswith("gfx_")]
+        return watts, float(np.mean(clks))
+    except Exception:
+        return None
+
+
+def power_leg(torch, work, seconds=1.5, skip=2):
+    """Board power and gfx clock while `work` runs back to back for about
+    `seconds` (a sampler thread calls amd-smi, ~5 samples/s; the first `skip`
+    samples, taken while the clock ramps, are dropped): the median of each."""
+    samples, stop = [], threading.Event()
+
+    def sampler():
+        while not stop.is_set():
+            s = _smi_sample()
+            if s is None:
+                return
+            samples.append(s)
+
+    th = threading.Thread(target=sampler)
+    th.start()
+    t_end, k = time.perf_counter() + seconds, 0
+    while time.perf_counter() < t_end:
+        work()
+        k += 1
+        if k % 16 == 0:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    stop.set()
+    th.join()
+    kept = samples[skip:] if len(samples) > skip else samples
+    if not kept:
+        return None
+    return {"watts": float(np.median([s[0] for s in kept])), "gfx_mhz": float(np.median([s[1] for s in kept])),
+            "samples": len(kept)}
+
+
 def schedule_peaks(ax, seconds=0.3):
     """Register-resident butterfly rate of every pass schedule: the
     microbenchmark back to back for ~`seconds`, median of the second half of
@@ -371,6 +419,22 @@ def kernel_roofline(info, ms, peaks):
     bound = "valu" if valu["frac"] >= hbm["frac"] else "hbm"
     return {"ms": ms, "info": info, "valu": valu, "hbm": hbm, "bound": bound,
             "frac": max(valu["frac"], hbm["frac"])}
+
+
+def valu_at_clock(kd, power):
+    """The dominant pass's VALU fraction against its schedule's peak scaled
+    to the gfx clock the board holds during the step: the microbenchmark runs
+    at its own (higher) clock, while the step, with HBM streaming beside the
+    arithmetic, runs at the 1400 W board limit and a lower clock."""
+    if not power or "error" in power or not power.get("butterfly_peak"):
+        return None
+    f_step, f_peak = power["ntt_step"]["gfx_mhz"], power["butterfly_peak"]["gfx_mhz"]
+    if f_step <= 0 or f_peak <= 0:
+        return None
+    peak = kd["valu"]["peak"] * f_step / f_peak
+    return {"achieved": kd["valu"]["achieved"], "peak": peak, "unit": "Gbfly/s",
+            "frac": kd["valu"]["achieved"] / peak, "step_gfx_mhz": f_step, "peak_gfx_mhz": f_peak,
+            "step_watts": power["ntt_step"]["watts"], "peak_watts": power["butterfly_peak"]["watts"]}
 
 
 def mix_roofline(infos, ms, peaks):
@@ -416,6 +480,7 @@ def main():
     ap.add_argument("--no-polymul", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-lazy", action="store_true")
+    ap.add_argument("--no-power", action="store_true", help="skip the board power and clock samples")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run one 2^27-word EltwiseReduceMod(4->2) (1 GiB in, 1 GiB out): "
@@ -518,6 +583,27 @@ def main():
     log(f"dominant pass {dom} ({kd['info']['schedule_name']}): {kd['valu']['achieved']:.1f} of "
         f"{kd['valu']['peak']:.1f} Gbfly/s = {kd['valu']['frac']:.3f}; {kd['hbm']['achieved']:.0f} GB/s = "
         f"{kd['hbm']['frac']:.3f} of HBM; bound {kd['bound']}")
+
+    # board power and gfx clock: the headline step, the dominant pass's
+    # butterfly microbenchmark (its peak's own clock) and a plain HBM stream
+    power = None
+    if not args.no_power:
+        flat_s, flat_a = src.view(-1), aux.view(-1)
+        words = flat_s.numel()
+        power = {
+            "ntt_step": power_leg(torch, launch),
+            "butterfly_peak": power_leg(torch, lambda: ax.butterfly_peak(kd["info"]["schedule"])),
+            "hbm_stream": power_leg(torch, lambda: ax.EltwiseReduceMod(flat_a, flat_s, words, 0x0FFFFFFFFFFC0001,
+                                                                       4, 2)),
+            "board_power_limit_w": 1400,
+            "source": "amd-smi metric -p -c (socket power, mean gfx clock over the XCDs), median of the samples "
+                      "taken while each workload runs back to back for ~1.5 s; 1400 W is MI355X's published "
+                      "total board power",
+        }
+        if power["ntt_step"] is None:
+            power = {"error": "amd-smi not available"}
+        else:
+            log("power:", json.dumps(power))
 
     # lazy transforms (SEAL's hot calls: ComputeForward(.., 4, 4) and
     # ComputeInverse(.., 2, 2)) and the normalized inverse, as batches
@@ -645,7 +731,9 @@ def main():
                      "rule": "larger of the VALU fraction (butterflies/s over the register-resident peak of the "
                              "pass's own butterfly schedule) and the HBM fraction (16 N bytes per limb per pass "
                              "over 8 TB/s)",
-                     "valu": kd["valu"], "hbm": kd["hbm"], "valu_issue": valu_issue},
+                     "valu": kd["valu"], "hbm": kd["hbm"], "valu_issue": valu_issue,
+                     "valu_at_step_clock": valu_at_clock(kd, power)},
+        "power": power,
         "eltwise": elt,
         "polymul": pmul,
         "uneven_batch": uneven,
