@@ -93,7 +93,7 @@ class ServeConfig(ctypes.Structure):
     """axhel_hip_serve_config (include/axhel_hip.h)."""
     _fields_ = [("memory_budget_bytes", _u64), ("max_batch_limbs", ctypes.c_uint32), ("slots", ctypes.c_uint32),
                 ("backend", ctypes.c_uint32), ("null_latency_us", ctypes.c_uint32), ("fail_every_alloc", _u64),
-                ("fail_every_batch", _u64)]
+                ("fail_every_batch", _u64), ("latency_samples", _u64)]
 
 
 class Request(ctypes.Structure):
@@ -533,9 +533,9 @@ class Server:
     or PENDING when wait's own timeout passed first (the ticket stays valid)."""
 
     def __init__(self, memory_budget_bytes=0, max_batch_limbs=0, slots=0, backend=BACKEND_DEVICE,
-                 null_latency_us=0, fail_every_alloc=0, fail_every_batch=0):
+                 null_latency_us=0, fail_every_alloc=0, fail_every_batch=0, latency_samples=0):
         cfg = ServeConfig(memory_budget_bytes, max_batch_limbs, slots, backend, null_latency_us, fail_every_alloc,
-                          fail_every_batch)
+                          fail_every_batch, latency_samples)
         h = _p()
         _check(_load().axhel_hip_server_create(ctypes.byref(cfg), ctypes.byref(h)), "server_create")
         self._h = h

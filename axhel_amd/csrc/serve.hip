@@ -75,6 +75,38 @@ struct Slot {
 
 bool is_ntt(int op) { return op == AXHEL_HIP_OP_NTT_FORWARD || op == AXHEL_HIP_OP_NTT_INVERSE; }
 
+// Latencies of completed requests in bounded memory: every sample up to cap
+// (2^20 by default, 8 MiB), then a uniform sample of cap of them (reservoir
+// sampling, Vitter's Algorithm R) from which the percentiles are read; count,
+// sum and max stay exact.  A server that runs for days keeps its memory.
+struct LatencyRecord {
+    size_t cap = size_t(1) << 20;
+    std::vector<double> sample;
+    uint64_t count = 0;
+    double sum = 0, max = 0;
+    uint64_t rng = 0x9E3779B97F4A7C15ull;
+
+    void add(double us) {
+        ++count;
+        sum += us;
+        max = std::max(max, us);
+        if (sample.size() < cap) {
+            sample.push_back(us);
+            return;
+        }
+        rng ^= rng << 13;  // xorshift64
+        rng ^= rng >> 7;
+        rng ^= rng << 17;
+        const uint64_t j = rng % count;
+        if (j < cap) sample[j] = us;
+    }
+    void clear() {
+        sample.clear();
+        count = 0;
+        sum = max = 0;
+    }
+};
+
 }  // namespace
 
 struct axhel_hip_server {
@@ -100,7 +132,7 @@ struct axhel_hip_server {
     std::map<std::tuple<uint64_t, uint64_t, uint64_t>, axhel_hip_plan *> plans;
     // stats
     axhel_hip_serve_stats st{};
-    std::vector<double> lat_us;
+    LatencyRecord lat;
     bool have_first = false;
     Clock::time_point first_arrival, last_done;
 };
@@ -316,8 +348,7 @@ void finish_req(axhel_hip_server *s, Req *R, axhel_hip_status status) {
         case AXHEL_HIP_OK: {
             ++s->st.completed;
             s->st.limbs_completed += R->r.limbs;
-            const double us = std::chrono::duration<double, std::micro>(R->done - R->arrival).count();
-            s->lat_us.push_back(us);
+            s->lat.add(std::chrono::duration<double, std::micro>(R->done - R->arrival).count());
             s->last_done = R->done;
             break;
         }
@@ -512,6 +543,7 @@ axhel_hip_status axhel_hip_server_create(const axhel_hip_serve_config *config, a
     if (config) s->cfg = *config;
     if (s->cfg.max_batch_limbs == 0) s->cfg.max_batch_limbs = 8192;
     if (s->cfg.slots == 0) s->cfg.slots = 4;  // one stream per hardware queue of a process (HIP's default 4)
+    if (s->cfg.latency_samples) s->lat.cap = (size_t)std::min<uint64_t>(s->cfg.latency_samples, 1ull << 28);
     s->device = s->cfg.backend == AXHEL_HIP_BACKEND_DEVICE;
     if (s->device && axhel_hip_device_count() == 0) {
         delete s;
@@ -690,15 +722,10 @@ axhel_hip_status axhel_hip_server_stats(axhel_hip_server *s, axhel_hip_serve_sta
     out->bytes_in_use = s->in_use;
     out->peak_bytes_in_use = s->peak;
     out->descriptor_bytes = s->desc_bytes;
-    out->p50_us = percentile(s->lat_us, 0.50);
-    out->p99_us = percentile(s->lat_us, 0.99);
-    double sum = 0, mx = 0;
-    for (double x : s->lat_us) {
-        sum += x;
-        mx = std::max(mx, x);
-    }
-    out->mean_us = s->lat_us.empty() ? 0 : sum / (double)s->lat_us.size();
-    out->max_us = mx;
+    out->p50_us = percentile(s->lat.sample, 0.50);
+    out->p99_us = percentile(s->lat.sample, 0.99);
+    out->mean_us = s->lat.count ? s->lat.sum / (double)s->lat.count : 0;
+    out->max_us = s->lat.max;
     out->first_arrival_to_last_completion_s =
         s->have_first && s->st.completed ? std::chrono::duration<double>(s->last_done - s->first_arrival).count()
                                          : 0.0;
@@ -709,7 +736,7 @@ axhel_hip_status axhel_hip_server_reset_stats(axhel_hip_server *s) {
     if (!s) return AXHEL_HIP_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lk(s->mu);
     s->st = axhel_hip_serve_stats{};
-    s->lat_us.clear();
+    s->lat.clear();
     s->have_first = false;
     s->peak = s->in_use + s->desc_bytes;
     return AXHEL_HIP_OK;

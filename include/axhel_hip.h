@@ -271,6 +271,7 @@ typedef struct axhel_hip_serve_config {
     uint64_t fail_every_alloc;    /* fault injection: every k-th staging allocation fails; 0 = never */
     uint64_t fail_every_batch;    /* fault injection: every k-th device batch fails after its staging copies
                                      are queued (its requests end AXHEL_HIP_NO_MEMORY); 0 = never */
+    uint64_t latency_samples;     /* latencies kept for p50/p99 (a uniform sample beyond); 0 = 2^20 */
 } axhel_hip_serve_config;
 
 /* One request: `limbs` vectors of `degree` words, limb l at offset l*degree
@@ -297,7 +298,11 @@ typedef struct axhel_hip_serve_stats {
     uint64_t limbs_completed, batches;
     uint64_t bytes_in_use;      /* staging of admitted requests */
     uint64_t peak_bytes_in_use; /* peak of staging plus descriptor buffers */
-    double p50_us, p99_us, mean_us, max_us; /* arrival -> completion, completed requests */
+    /* arrival -> completion of the requests completed since the last reset;
+       p50/p99 exact up to latency_samples of them, then read from a uniform
+       sample of that many (the server's memory stays bounded); mean and max
+       exact */
+    double p50_us, p99_us, mean_us, max_us;
     double first_arrival_to_last_completion_s;
     uint64_t descriptor_bytes; /* the slots' device descriptor buffers (charged to the budget) */
 } axhel_hip_serve_stats;

@@ -184,3 +184,29 @@ def test_shutdown_cancels_queued_and_finishes_running():
     th.join(timeout=5)
     assert out.get("b") == ax.CANCELLED
     s._h = ax._p()
+
+
+def test_latency_record_is_bounded():
+    """Past `latency_samples` completions the percentiles come from a uniform
+    sample of that many latencies (the server's memory stays bounded), while
+    count, mean and max stay exact: 400 requests through a record of 16."""
+    s = ax.Server(backend=ax.BACKEND_NULL, null_latency_us=50, memory_budget_bytes=256 * MIB, latency_samples=16)
+    keep, tickets = [], []
+    for i in range(400):
+        st, t, bufs = req(s)
+        assert st == ax.OK
+        tickets.append(t)
+        keep.append(bufs)
+        if len(tickets) == 50:
+            for t in tickets:
+                assert s.wait(t) == ax.OK
+            tickets, keep = [], []
+    for t in tickets:
+        assert s.wait(t) == ax.OK
+    stt = s.stats()
+    assert stt["completed"] == 400
+    assert 0 < stt["p50_us"] <= stt["p99_us"] <= stt["max_us"]
+    assert 0 < stt["mean_us"] <= stt["max_us"]
+    s.reset_stats()
+    assert s.stats()["completed"] == 0 and s.stats()["max_us"] == 0
+    s.close()
