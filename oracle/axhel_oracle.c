@@ -13,9 +13,15 @@
  *   tables            example/example.cpp:315-345 golden N=8 tables; closed form
  *   normalized NTT    tests/public/test_ntt.cpp:32-46 KAT; O(N^2) DFT closed form
  *   lazy NTT          congruence with the closed form at every stage, the lazy
- *                     ranges [0,4q)/[0,2q), the Harvey pair-sum identity, the
- *                     Shoup representative from its big-integer definition.
- *                     No reference-held vector exists for lazy bit patterns.
+ *                     ranges [0,4q)/[0,2q), the Shoup representative from its
+ *                     big-integer definition, and the lazy words themselves
+ *                     against Harvey 2014 Algorithm 4 and its Gentleman-Sande
+ *                     counterpart restated over Python integers (twiddles
+ *                     from psi, boundary words 0, q, 2q-1, 2q, 4q-1); the last
+ *                     inverse stage's representative follows the reference's
+ *                     fused scaling (ntt-native.cpp:101-141), restated there
+ *                     too.  No reference-held vector exists for lazy bit
+ *                     patterns.
  *   add/sub/mul/fma   closed form (a op b) mod q, exhaustive small q, example KATs
  *   reduce            closed form; Barrett mode KAT example/example.cpp:274-285
  *   mt19937_64        the C++ standard's 10000th-output check value

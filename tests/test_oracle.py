@@ -168,6 +168,133 @@ def negacyclic(a, b, q):
     return out
 
 
+# ----------------------------------------- lazy representatives (Harvey 2014)
+#
+# The checks above fix every lazy word modulo q and its range.  Which
+# congruent value a lazy transform returns is fixed by the butterfly itself:
+# D. Harvey, "Faster arithmetic for number-theoretic transforms", J. Symbolic
+# Computation 60 (2014) 113-119, Algorithm 4, the butterfly the reference
+# names and implements (ntt-internal.hpp:20-31), and its Gentleman-Sande
+# counterpart (ntt-internal.hpp:37-49).  Restated below over Python integers
+# from the paper's statement, with every twiddle and Shoup quotient computed
+# here from psi (not read from the oracle's tables); the last inverse stage
+# is the reference's own fused scaling (ntt-native.cpp:101-141), which no
+# publication fixes.  The oracle must agree word for word, lazy values
+# included, on inputs that include the boundary words 0, q, 2q - 1, 2q and
+# 4q - 1.
+
+def _shoup_quot(w, q):
+    return (w << 64) // q
+
+
+def _shoup_exact(y, w, wq, q):
+    """Harvey's Algorithm 4 lines 2-3: T = W Y - floor(W' Y / 2^64) p, an
+    integer in [0, 2p) (Harvey's Lemma 2 for Y < 2^64)."""
+    t = w * y - ((wq * y) >> 64) * q
+    assert 0 <= t < 2 * q
+    return t
+
+
+def harvey_forward(a, q, psi):
+    """Negacyclic Cooley-Tukey transform, bit-reversed output, with Harvey's
+    lazy butterfly: inputs and outputs in [0, 4p)."""
+    x = [int(v) for v in a]
+    n = len(x)
+    logn = n.bit_length() - 1
+    m, gap = 1, n >> 1
+    while m < n:
+        for i in range(m):
+            w = pow(psi, brv(m + i, logn), q)  # the twiddle of block i when there are m blocks
+            wq = _shoup_quot(w, q)
+            for j in range(i * 2 * gap, i * 2 * gap + gap):
+                X, Y = x[j], x[j + gap]
+                if X >= 2 * q:  # Algorithm 4 line 1
+                    X -= 2 * q
+                T = _shoup_exact(Y, w, wq, q)
+                x[j], x[j + gap] = X + T, X - T + 2 * q
+        m, gap = m << 1, gap >> 1
+    return x
+
+
+def gs_inverse(a, q, psi, final=True):
+    """Gentleman-Sande inverse on [0, 2p) words with the lazy butterfly
+    X' = X + Y reduced below 2p, Y' = W (X - Y + 2p) by Shoup; then the
+    reference's last stage, which folds n^-1 into both outputs."""
+    x = [int(v) for v in a]
+    n = len(x)
+    logn = n.bit_length() - 1
+    ipsi = pow(psi, q - 2, q)
+    m, gap = n >> 1, 1
+    while m > 1:
+        for i in range(m):
+            w = pow(ipsi, brv(n - 2 * m + i, logn) + 1, q)  # table entry n - 2m + 1 + i
+            wq = _shoup_quot(w, q)
+            for j in range(i * 2 * gap, i * 2 * gap + gap):
+                X, Y = x[j], x[j + gap]
+                S = X + Y
+                if S >= 2 * q:
+                    S -= 2 * q
+                x[j], x[j + gap] = S, _shoup_exact(X - Y + 2 * q, w, wq, q)
+        m, gap = m >> 1, gap << 1
+    if not final:
+        return x
+    ninv = pow(n, q - 2, q)
+    ninv_q = _shoup_quot(ninv, q)
+    wl = pow(ipsi, brv(n - 2, logn) + 1, q)  # table entry n - 1
+    s = _shoup_exact(wl, ninv, ninv_q, q)
+    if s >= q:
+        s -= q
+    sq = _shoup_quot(s, q)
+    for j in range(gap):
+        X, Y = x[j], x[j + gap]
+        if X >= 2 * q:
+            X -= 2 * q
+        x[j], x[j + gap] = _shoup_exact(X + Y, ninv, ninv_q, q), _shoup_exact(X + 2 * q - Y, s, sq, q)
+    return x
+
+
+def _boundary_words(q, bound, n, seed):
+    rng = random.Random(seed)
+    edges = [0, 1, q - 1, q, 2 * q - 1, 2 * q, 4 * q - 1]
+    words = [e for e in edges if e < bound]
+    return [words[i] if i < len(words) else rng.randrange(bound) for i in range(n)]
+
+
+@pytest.mark.parametrize("n,bits", [(2, 20), (8, 40), (64, 50), (256, 62), (1024, 60)])
+def test_forward_lazy_representative_harvey(n, bits):
+    q = params.find_ntt_prime(n, bits)
+    psi = params.find_primitive_2n_root(n, q)
+    t = O.Tables(n, q, psi)
+    for seed in range(3):
+        a = _boundary_words(q, 4 * q, n, seed * 1000 + n)
+        if seed == 1:
+            a = a[::-1]  # boundary words on the butterflies' other operand
+        if seed == 2:
+            a = [2 * q if i % 2 == 0 else v for i, v in enumerate(a)]  # x == 2q exactly in every first-stage pair
+        got = O.compute_forward(np.array(a, dtype=np.uint64), t, 4, 4)
+        assert got.tolist() == harvey_forward(a, q, psi), seed
+
+
+@pytest.mark.parametrize("n,bits", [(2, 20), (8, 40), (64, 50), (256, 62), (1024, 60)])
+def test_inverse_lazy_representative_harvey(n, bits):
+    q = params.find_ntt_prime(n, bits)
+    psi = params.find_primitive_2n_root(n, q)
+    t = O.Tables(n, q, psi)
+    logn = n.bit_length() - 1
+    for seed in range(3):
+        a = _boundary_words(q, 2 * q, n, seed * 1000 + n)
+        if seed == 1:
+            a = a[::-1]
+        if seed == 2:
+            a = [q] * n  # X + Y == 2q exactly in every first-stage pair
+        arr = np.array(a, dtype=np.uint64)
+        if logn > 1:
+            mid = O.inv_lazy(arr, t, stages=logn - 1, final=False)
+            assert mid.tolist() == gs_inverse(a, q, psi, final=False), seed
+        got = O.compute_inverse(arr, t, 2, 2)
+        assert got.tolist() == gs_inverse(a, q, psi), seed
+
+
 @pytest.mark.parametrize("n,bits", [(8, 40), (32, 60), (64, 62)])
 def test_polymul_matches_quadratic_oracle(n, bits):
     """tests/public/test_polymul.cpp:13-25 call sequence vs O(N^2) product."""

@@ -6,11 +6,11 @@ least one of the checks in tests/test_oracle.py that pin the real oracle
 (the reference's golden vectors, closed forms, round trips and range
 invariants); the unmutated source, compiled the same way, passes them all.
 
-Slips that change only which congruent value a lazy transform returns,
-while staying in its range, are not in this list: no reference-held vector
-and no invariant fixes the lazy representative, which is pinned by the
-oracle following the reference's loop statement by statement.  CPU only;
-nothing here runs on a GPU."""
+The last three mutants change only which congruent value a lazy transform
+returns.  The first two of them pass every congruence and range check and
+fail only the restatement of Harvey's butterflies (tests/test_oracle.py,
+the *_lazy_representative_harvey checks); the third also breaks the
+inverse round trip.  CPU only; nothing here runs on a GPU."""
 import importlib.util
 import os
 import subprocess
@@ -66,6 +66,14 @@ MUTANTS = [
      "if (out_f != 4) oracle_reduce(res, res, n, q, 4, 2);"),
     ("mul: factor 8 accepted instead of a no-op",
      "if (f != 1 && f != 2 && f != 4) return;", "if (f != 1 && f != 2 && f != 4 && f != 8) return;"),
+    # representative-only slips: same residues, same ranges, other lazy words
+    ("fwd (representative): 2q subtracted only above 2q",
+     "uint64_t xh = csub(x[j], q2);                 /* [0,4q) -> [0,2q) */",
+     "uint64_t xh = x[j] > q2 ? x[j] - q2 : x[j];"),
+    ("inv (representative): sum reduced only above 2q",
+     "x[j] = csub(u + v, q2);", "x[j] = u + v > q2 ? u + v - q2 : u + v;"),
+    ("inv final stage (representative): difference offset 4q",
+     "uint64_t dif = xh + q2 - yv;", "uint64_t dif = xh + 2 * q2 - yv;"),
 ]
 
 # the pinning checks of tests/test_oracle.py, at sizes that run in well under a second
@@ -87,6 +95,10 @@ CHECKS = [
     (T.test_reduce_closed_form, dict(bits=50)),
     (T.test_reduce_closed_form, dict(bits=62)),
     (T.test_unsupported_factors_are_noops, {}),
+    (T.test_forward_lazy_representative_harvey, dict(n=2, bits=20)),
+    (T.test_forward_lazy_representative_harvey, dict(n=64, bits=50)),
+    (T.test_inverse_lazy_representative_harvey, dict(n=2, bits=20)),
+    (T.test_inverse_lazy_representative_harvey, dict(n=64, bits=50)),
 ]
 
 
