@@ -316,6 +316,31 @@ struct TwLdsT {
     }
 };
 
+// Forward passes: the thread's index bits lie below the register bits of
+// chunk 0 (Geom::kt(0, tid) < 2^LO(0)), so on every stage of chunk 0 the
+// entry (kt >> (p+1)) + pr is the compile-time pr alone.  Where the table
+// range is also the same for the whole workgroup (a COL pass: one limb per
+// workgroup; the stacked ROW pass: one row o and one plan per workgroup),
+// those entries are read through U with kt = 0: a uniform address, which
+// the compiler serves with scalar loads instead of a per-lane LDS read each.
+// Later chunks read through T.
+template <typename U, typename T>
+struct TwChunk0Uniform {
+    static constexpr bool kChunked = true;
+    U u;
+    T t;
+    template <int C, int P>
+    __device__ __forceinline__ ulonglong2 getc(uint32_t kt, uint32_t pr) const {
+        if constexpr (C == 0) return u.template get<P>(0u, pr);
+        else return t.template get<P>(kt, pr);
+    }
+};
+
+template <typename TW, typename = void>
+struct tw_chunked : std::false_type {};
+template <typename TW>
+struct tw_chunked<TW, std::void_t<decltype(TW::kChunked)>> : std::bool_constant<TW::kChunked> {};
+
 // Staging of a pass's 2^LOG_G - 1 twiddle pairs into LDS by NT threads:
 // fetch() issues the global loads into registers (K per thread), so the
 // caller can issue its data loads before store() waits for them; the
@@ -471,7 +496,13 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
                     constexpr uint32_t pr = Gm::kr(c, r) >> (p + 1);
                     if constexpr (rp == r) {
                         if (!fin) {
-                            const ulonglong2 tv = tws.template get<p>(ktid, pr);
+                            ulonglong2 tv;
+                            if constexpr (tw_chunked<TW>::value) {
+                                static_assert(FWD, "chunk-0 entries are uniform in forward passes only");
+                                tv = tws.template getc<c, p>(ktid, pr);
+                            } else {
+                                tv = tws.template get<p>(ktid, pr);
+                            }
                             w[r] = tv.x;
                             wq[r] = tv.y;
                         }
@@ -626,8 +657,14 @@ ntt_pass_kernel(PassArgs a) {
     if constexpr (COL) {
         tstage.store(tw_lds, t);
         __syncthreads();
-        const TwLds<LOG_G, FWD> tws{tw_lds};
-        run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
+        if constexpr (FWD) {
+            // one limb per workgroup: chunk 0's entries are uniform
+            const TwChunk0Uniform<TwGlobal<LOG_G, true, true>, TwLds<LOG_G, true>> tws{{tw, n, blo, o}, {tw_lds}};
+            run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
+        } else {
+            const TwLds<LOG_G, FWD> tws{tw_lds};
+            run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
+        }
     } else {
         // a ROW group's exchanges touch its own LDS words only (the staged
         // loads above and the staged stores below are published by
@@ -789,8 +826,15 @@ ntt_row_stack_kernel(PassArgs a) {
     // a row's LDS words are its own: when a row lies inside one wave, its
     // exchanges and its store staging synchronize that wave alone
     constexpr bool WS = row_group_in_wave<LOG_G, LOG_E>();
-    const TwLdsT<LOG_G, LOG_E, FWD> tws{tw_lds};
-    run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX, WS>(v, lds, g, tid, m, tws, false, J0);
+    if constexpr (FWD) {
+        // one row o and one plan per workgroup: chunk 0's entries are uniform
+        const TwChunk0Uniform<TwGlobal<LOG_G, true, false>, TwLdsT<LOG_G, LOG_E, true>> tws{{J0.tw, n, 0u, o},
+                                                                                           {tw_lds}};
+        run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX, WS>(v, lds, g, tid, m, tws, false, J0);
+    } else {
+        const TwLdsT<LOG_G, LOG_E, FWD> tws{tw_lds};
+        run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX, WS>(v, lds, g, tid, m, tws, false, J0);
+    }
 
     if (a.normalize) {
         static_for<0, E>([&](auto rc) {

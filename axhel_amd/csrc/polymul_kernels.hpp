@@ -208,7 +208,9 @@ polymul_row_stack_kernel(PolyArgs a) {
     sf.store(twf_lds, t);
     si.store(twi_lds, t);
     __syncthreads();  // publishes the twiddles
-    const TwLdsT<LOG_G, LOG_E, true> twf{twf_lds};
+    // one row o and one plan per workgroup: chunk 0's forward entries are uniform
+    const TwChunk0Uniform<TwGlobal<LOG_G, true, false>, TwLdsT<LOG_G, LOG_E, true>> twf{{P0.twf, n, 0u, o},
+                                                                                        {twf_lds}};
     const TwLdsT<LOG_G, LOG_E, false> twi{twi_lds};
     // the staged twiddles are shared (published above); every other LDS word
     // a row touches is its own, so a row inside one wave synchronizes alone
