@@ -316,25 +316,33 @@ struct TwLdsT {
     }
 };
 
-// Forward passes: the thread's index bits lie below the register bits of
-// chunk 0 (Geom::kt(0, tid) < 2^LO(0)), so on every stage of chunk 0 the
-// entry (kt >> (p+1)) + pr is the compile-time pr alone.  Where the table
-// range is also the same for the whole workgroup (a COL pass: one limb per
+// The chunk that holds a pass's top bits (a forward pass's chunk 0, an
+// inverse pass's last chunk) has the thread's index bits below its register
+// bits (Geom::kt(c, tid) < 2^LO(c)), so on each of its stages the entry
+// (kt >> (p+1)) + pr is the compile-time pr alone.  Where the table range is
+// also the same for the whole workgroup (a COL pass: one limb per
 // workgroup; the stacked ROW pass: one row o and one plan per workgroup),
-// those entries are read through U with kt = 0: a uniform address, which
-// the compiler serves with scalar loads instead of a per-lane LDS read each.
-// Later chunks read through T.
-template <typename U, typename T>
-struct TwChunk0Uniform {
+// that chunk's entries (chunk UC) are read through U with kt = 0: a uniform
+// address, which the compiler serves with scalar loads instead of a
+// per-lane LDS read each.  The other chunks read through T.
+template <int UC, typename U, typename T>
+struct TwTopChunkUniform {
     static constexpr bool kChunked = true;
+    static constexpr int kUniformChunk = UC;
     U u;
     T t;
     template <int C, int P>
     __device__ __forceinline__ ulonglong2 getc(uint32_t kt, uint32_t pr) const {
-        if constexpr (C == 0) return u.template get<P>(0u, pr);
+        if constexpr (C == UC) return u.template get<P>(0u, pr);
         else return t.template get<P>(kt, pr);
     }
 };
+
+// The chunk that holds the top bits of a pass with LOG_G bits, LOG_E per chunk.
+template <int LOG_G, int LOG_E, bool FWD>
+constexpr int top_chunk() {
+    return FWD ? 0 : Geom<LOG_G, LOG_E, FWD>::NC - 1;
+}
 
 template <typename TW, typename = void>
 struct tw_chunked : std::false_type {};
@@ -498,7 +506,8 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
                         if (!fin) {
                             ulonglong2 tv;
                             if constexpr (tw_chunked<TW>::value) {
-                                static_assert(FWD, "chunk-0 entries are uniform in forward passes only");
+                                static_assert(Gm::LO(TW::kUniformChunk) + Gm::B(TW::kUniformChunk) == LOG_G,
+                                              "only the top chunk's entries are uniform");
                                 tv = tws.template getc<c, p>(ktid, pr);
                             } else {
                                 tv = tws.template get<p>(ktid, pr);
@@ -657,14 +666,10 @@ ntt_pass_kernel(PassArgs a) {
     if constexpr (COL) {
         tstage.store(tw_lds, t);
         __syncthreads();
-        if constexpr (FWD) {
-            // one limb per workgroup: chunk 0's entries are uniform
-            const TwChunk0Uniform<TwGlobal<LOG_G, true, true>, TwLds<LOG_G, true>> tws{{tw, n, blo, o}, {tw_lds}};
-            run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
-        } else {
-            const TwLds<LOG_G, FWD> tws{tw_lds};
-            run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
-        }
+        // one limb per workgroup: the top chunk's entries are uniform
+        const TwTopChunkUniform<top_chunk<LOG_G, LOG_E, FWD>(), TwGlobal<LOG_G, FWD, true>, TwLds<LOG_G, FWD>> tws{
+            {tw, n, blo, o}, {tw_lds}};
+        run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
     } else {
         // a ROW group's exchanges touch its own LDS words only (the staged
         // loads above and the staged stores below are published by
@@ -826,15 +831,10 @@ ntt_row_stack_kernel(PassArgs a) {
     // a row's LDS words are its own: when a row lies inside one wave, its
     // exchanges and its store staging synchronize that wave alone
     constexpr bool WS = row_group_in_wave<LOG_G, LOG_E>();
-    if constexpr (FWD) {
-        // one row o and one plan per workgroup: chunk 0's entries are uniform
-        const TwChunk0Uniform<TwGlobal<LOG_G, true, false>, TwLdsT<LOG_G, LOG_E, true>> tws{{J0.tw, n, 0u, o},
-                                                                                           {tw_lds}};
-        run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX, WS>(v, lds, g, tid, m, tws, false, J0);
-    } else {
-        const TwLdsT<LOG_G, LOG_E, FWD> tws{tw_lds};
-        run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX, WS>(v, lds, g, tid, m, tws, false, J0);
-    }
+    // one row o and one plan per workgroup: the top chunk's entries are uniform
+    const TwTopChunkUniform<top_chunk<LOG_G, LOG_E, FWD>(), TwGlobal<LOG_G, FWD, false>, TwLdsT<LOG_G, LOG_E, FWD>> tws{
+        {J0.tw, n, 0u, o}, {tw_lds}};
+    run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX, WS>(v, lds, g, tid, m, tws, false, J0);
 
     if (a.normalize) {
         static_for<0, E>([&](auto rc) {

@@ -208,10 +208,12 @@ polymul_row_stack_kernel(PolyArgs a) {
     sf.store(twf_lds, t);
     si.store(twi_lds, t);
     __syncthreads();  // publishes the twiddles
-    // one row o and one plan per workgroup: chunk 0's forward entries are uniform
-    const TwChunk0Uniform<TwGlobal<LOG_G, true, false>, TwLdsT<LOG_G, LOG_E, true>> twf{{P0.twf, n, 0u, o},
-                                                                                        {twf_lds}};
-    const TwLdsT<LOG_G, LOG_E, false> twi{twi_lds};
+    // one row o and one plan per workgroup: the top chunk's entries are uniform
+    const TwTopChunkUniform<top_chunk<LOG_G, LOG_E, true>(), TwGlobal<LOG_G, true, false>, TwLdsT<LOG_G, LOG_E, true>>
+        twf{{P0.twf, n, 0u, o}, {twf_lds}};
+    const TwTopChunkUniform<top_chunk<LOG_G, LOG_E, false>(), TwGlobal<LOG_G, false, false>,
+                            TwLdsT<LOG_G, LOG_E, false>>
+        twi{{P0.twi, n, 0u, o}, {twi_lds}};
     // the staged twiddles are shared (published above); every other LDS word
     // a row touches is its own, so a row inside one wave synchronizes alone
     constexpr bool WS = row_group_in_wave<LOG_G, LOG_E>();
