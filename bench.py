@@ -424,8 +424,8 @@ def kernel_roofline(info, ms, peaks):
 def valu_at_clock(kd, power):
     """The dominant pass's VALU fraction against its schedule's peak scaled
     to the gfx clock the board holds during the step: the microbenchmark runs
-    at its own (higher) clock, while the step, with HBM streaming beside the
-    arithmetic, runs at the 1400 W board limit and a lower clock."""
+    at its own clock, while the step, with HBM streaming beside the
+    arithmetic, may run lower when the board reaches its 1400 W limit."""
     if not power or "error" in power or not power.get("butterfly_peak"):
         return None
     f_step, f_peak = power["ntt_step"]["gfx_mhz"], power["butterfly_peak"]["gfx_mhz"]
