@@ -109,7 +109,7 @@ class ServeStats(ctypes.Structure):
                                     "limbs_completed", "batches", "bytes_in_use", "peak_bytes_in_use")] + \
                [(k, ctypes.c_double) for k in ("p50_us", "p99_us", "mean_us", "max_us",
                                               "first_arrival_to_last_completion_s")] + \
-               [("descriptor_bytes", _u64)]
+               [("descriptor_bytes", _u64), ("pool_bytes", _u64), ("peak_pool_bytes", _u64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

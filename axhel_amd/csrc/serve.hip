@@ -9,7 +9,10 @@
 // slot's batch runs, newly arrived requests are admitted into the next free
 // slot, so work from later arrivals overlaps on the device.  The scheduler
 // observes completion per slot (event query, polled every 20 us while a
-// batch is in flight) and releases the batch's memory.  Its thread runs with
+// batch is in flight) and releases the batch's memory.  Released staging
+// blocks stay cached for later requests; the pool's whole footprint (blocks
+// in use and cached) plus the descriptor buffers is kept within the memory
+// budget by freeing cached blocks before a new one is allocated.  Its thread runs with
 // a 1 us timer slack: with Linux's default 50 us slack each 20 us poll slept
 // ~70 us, which showed in every request's latency (p50 at half load 84 ->
 // 68 us when the slack was cut, measured side by side).  Waiter threads
@@ -49,7 +52,8 @@ struct Req {
     std::vector<const axhel_hip_plan *> plans;
     std::vector<uint64_t> moduli, scalars;
     uint64_t words = 0;
-    uint64_t bytes = 0;  // staging bytes charged to the budget
+    uint64_t bytes = 0;  // staging bytes the request needs (admission)
+    uint64_t held = 0;   // bytes of the pool blocks it holds (a reused block may be larger)
     bool h_op1 = false, h_op2 = false, h_res = false;
     Clock::time_point arrival, done;
     int state = ST_QUEUED;
@@ -125,9 +129,10 @@ struct axhel_hip_server {
     std::thread worker;
     bool stopping = false;
     int active_waiters = 0;  // threads inside axhel_hip_server_wait
-    // pool: cached device blocks by size
+    // pool: cached device blocks by size; every block with its size
     std::multimap<uint64_t, void *> free_blocks;
-    std::vector<void *> all_blocks;
+    std::unordered_map<void *, uint64_t> all_blocks;
+    uint64_t pool_bytes = 0, pool_peak = 0;
     // plan cache
     std::map<std::tuple<uint64_t, uint64_t, uint64_t>, axhel_hip_plan *> plans;
     // stats
@@ -141,67 +146,97 @@ namespace {
 
 // ------------------------------------------------------------ staging pool
 
-hipError_t pool_alloc(axhel_hip_server *s, uint64_t bytes, void **p) {
+void pool_free_block(axhel_hip_server *s, void *b) {
+    if (s->device) (void)hipFree(b);
+    else std::free(b);
+}
+
+enum class Alloc { OK, OVER_BUDGET, FAILED };
+
+// A block of at least `bytes`: a cached one up to twice that size, else a
+// new one.  Before allocating, cached blocks are freed (largest first) until
+// the pool plus the descriptor buffers fit the budget with the new block;
+// none of them is read by queued work (a block returns to the cache when its
+// batch has completed).  If they still do not fit, OVER_BUDGET, unless
+// may_exceed.  hipFree waits for the device, so trimming stalls the
+// scheduler for the batches in flight; it happens only when the size mix
+// changes.  Called with the server mutex held.
+Alloc pool_alloc(axhel_hip_server *s, uint64_t bytes, bool may_exceed, void **p, uint64_t *got) {
     ++s->alloc_count;
-    if (s->cfg.fail_every_alloc && s->alloc_count % s->cfg.fail_every_alloc == 0) return hipErrorOutOfMemory;
+    if (s->cfg.fail_every_alloc && s->alloc_count % s->cfg.fail_every_alloc == 0) return Alloc::FAILED;
     auto it = s->free_blocks.lower_bound(bytes);
     if (it != s->free_blocks.end() && it->first <= bytes * 2) {
         *p = it->second;
+        *got = it->first;
         s->free_blocks.erase(it);
-        return hipSuccess;
+        return Alloc::OK;
     }
+    while (!s->free_blocks.empty() && s->pool_bytes + s->desc_bytes + bytes > s->budget) {
+        auto big = std::prev(s->free_blocks.end());
+        pool_free_block(s, big->second);
+        s->all_blocks.erase(big->second);
+        s->pool_bytes -= big->first;
+        s->free_blocks.erase(big);
+    }
+    if (!may_exceed && s->pool_bytes + s->desc_bytes + bytes > s->budget) return Alloc::OVER_BUDGET;
     void *d = nullptr;
     if (s->device) {
-        hipError_t e = hipMalloc(&d, bytes);
-        if (e != hipSuccess) return e;
+        if (hipMalloc(&d, bytes) != hipSuccess) return Alloc::FAILED;
     } else {
         d = std::malloc(bytes);
-        if (!d) return hipErrorOutOfMemory;
+        if (!d) return Alloc::FAILED;
     }
-    s->all_blocks.push_back(d);
+    s->all_blocks.emplace(d, bytes);
+    s->pool_bytes += bytes;
+    s->pool_peak = std::max(s->pool_peak, s->pool_bytes);
     *p = d;
-    return hipSuccess;
+    *got = bytes;
+    return Alloc::OK;
 }
 
 void pool_release(axhel_hip_server *s, Req *R) {
     for (auto &b : R->blocks) s->free_blocks.emplace(b.first, b.second);
     R->blocks.clear();
-    s->in_use -= R->bytes;
+    s->in_use -= R->held;
+    R->held = 0;
 }
 
-// Reserves staging for R's host buffers; false on allocation failure.
-bool stage_alloc(axhel_hip_server *s, Req *R) {
+// Reserves staging for R's host buffers.  On anything but OK the blocks
+// taken so far go back to the cache.
+Alloc stage_alloc(axhel_hip_server *s, Req *R, bool may_exceed) {
     const uint64_t wb = R->words * sizeof(uint64_t);
+    Alloc a = Alloc::OK;
     auto get = [&](void **p) -> bool {
         // block size rounded up to 1 MiB so blocks are reused across sizes
         const uint64_t sz = (wb + (1u << 20) - 1) & ~(uint64_t)((1u << 20) - 1);
-        if (pool_alloc(s, sz, p) != hipSuccess) return false;
-        R->blocks.emplace_back(sz, *p);
+        uint64_t got = 0;
+        if ((a = pool_alloc(s, sz, may_exceed, p, &got)) != Alloc::OK) return false;
+        R->blocks.emplace_back(got, *p);
+        R->held += got;
         return true;
     };
     void *p = nullptr;
     R->d_op1 = R->r.operand1;
     R->d_op2 = R->r.operand2;
     R->d_res = R->r.result;
-    if (R->h_op1) {
-        if (!get(&p)) return false;
-        R->d_op1 = (const uint64_t *)p;
-    }
-    if (R->h_op2) {
-        if (!get(&p)) return false;
-        R->d_op2 = (const uint64_t *)p;
-    }
-    if (R->h_res) {
+    bool ok = true;
+    if (R->h_op1 && (ok = get(&p))) R->d_op1 = (const uint64_t *)p;
+    if (ok && R->h_op2 && (ok = get(&p))) R->d_op2 = (const uint64_t *)p;
+    if (ok && R->h_res) {
         if (R->r.result == R->r.operand1 && R->h_op1) {
             R->d_res = const_cast<uint64_t *>(R->d_op1);
         } else if (R->r.result == R->r.operand2 && R->h_op2) {
             R->d_res = const_cast<uint64_t *>(R->d_op2);
-        } else {
-            if (!get(&p)) return false;
+        } else if ((ok = get(&p))) {
             R->d_res = (uint64_t *)p;
         }
     }
-    return true;
+    if (!ok) {
+        for (auto &b : R->blocks) s->free_blocks.emplace(b.first, b.second);
+        R->blocks.clear();
+        R->held = 0;
+    }
+    return a;
 }
 
 // ------------------------------------------------------------- device batch
@@ -424,14 +459,18 @@ void scheduler(axhel_hip_server *s) {
                 if (s->in_use + s->desc_bytes + R->bytes > s->budget && s->in_use > 0) break;
                 if (!free_slot->batch.empty() && limbs + R->r.limbs > s->cfg.max_batch_limbs) break;
                 s->waiting.pop_front();
-                if (!stage_alloc(s, R)) {
-                    for (auto &b : R->blocks) s->free_blocks.emplace(b.first, b.second);
-                    R->blocks.clear();
+                // the pool may pass the budget only for a request that runs alone
+                const Alloc a = stage_alloc(s, R, s->in_use == 0);
+                if (a == Alloc::OVER_BUDGET) {  // waits for running batches to return their blocks
+                    s->waiting.push_front(R);
+                    break;
+                }
+                if (a == Alloc::FAILED) {
                     finish_req(s, R, AXHEL_HIP_NO_MEMORY);
                     s->cv_done.notify_all();
                     continue;
                 }
-                s->in_use += R->bytes;
+                s->in_use += R->held;
                 s->peak = std::max(s->peak, s->in_use + s->desc_bytes);
                 R->state = ST_RUNNING;
                 free_slot->batch.push_back(R);
@@ -522,9 +561,9 @@ axhel_hip_status validate(const axhel_hip_request *r) {
     return AXHEL_HIP_INVALID_ARGUMENT;
 }
 
-double percentile(std::vector<double> v, double p) {
+// Linear interpolation between the closest ranks of a sorted sample.
+double percentile_sorted(const std::vector<double> &v, double p) {
     if (v.empty()) return 0.0;
-    std::sort(v.begin(), v.end());
     const double pos = p * (double)(v.size() - 1);
     const size_t i = (size_t)pos;
     const double f = pos - (double)i;
@@ -600,10 +639,7 @@ axhel_hip_status axhel_hip_server_destroy(axhel_hip_server *s) {
         if (sl.h_desc) (void)hipHostFree(sl.h_desc);
         if (sl.d_desc) (void)hipFree(sl.d_desc);
     }
-    for (void *b : s->all_blocks) {
-        if (s->device) (void)hipFree(b);
-        else std::free(b);
-    }
+    for (auto &kv : s->all_blocks) pool_free_block(s, kv.first);
     for (auto &kv : s->plans) axhel_hip_ntt_plan_destroy(kv.second);
     delete s;
     return AXHEL_HIP_OK;
@@ -717,18 +753,26 @@ axhel_hip_status axhel_hip_server_cancel(axhel_hip_server *s, uint64_t ticket) {
 
 axhel_hip_status axhel_hip_server_stats(axhel_hip_server *s, axhel_hip_serve_stats *out) {
     if (!s || !out) return AXHEL_HIP_INVALID_ARGUMENT;
-    std::lock_guard<std::mutex> lk(s->mu);
-    *out = s->st;
-    out->bytes_in_use = s->in_use;
-    out->peak_bytes_in_use = s->peak;
-    out->descriptor_bytes = s->desc_bytes;
-    out->p50_us = percentile(s->lat.sample, 0.50);
-    out->p99_us = percentile(s->lat.sample, 0.99);
-    out->mean_us = s->lat.count ? s->lat.sum / (double)s->lat.count : 0;
-    out->max_us = s->lat.max;
-    out->first_arrival_to_last_completion_s =
-        s->have_first && s->st.completed ? std::chrono::duration<double>(s->last_done - s->first_arrival).count()
-                                         : 0.0;
+    std::vector<double> sample;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        *out = s->st;
+        out->bytes_in_use = s->in_use;
+        out->peak_bytes_in_use = s->peak;
+        out->descriptor_bytes = s->desc_bytes;
+        out->pool_bytes = s->pool_bytes;
+        out->peak_pool_bytes = s->pool_peak;
+        out->mean_us = s->lat.count ? s->lat.sum / (double)s->lat.count : 0;
+        out->max_us = s->lat.max;
+        out->first_arrival_to_last_completion_s =
+            s->have_first && s->st.completed ? std::chrono::duration<double>(s->last_done - s->first_arrival).count()
+                                             : 0.0;
+        sample = s->lat.sample;
+    }
+    // sorted outside the lock: the scheduler does not wait for it
+    std::sort(sample.begin(), sample.end());
+    out->p50_us = percentile_sorted(sample, 0.50);
+    out->p99_us = percentile_sorted(sample, 0.99);
     return AXHEL_HIP_OK;
 }
 
@@ -739,6 +783,7 @@ axhel_hip_status axhel_hip_server_reset_stats(axhel_hip_server *s) {
     s->lat.clear();
     s->have_first = false;
     s->peak = s->in_use + s->desc_bytes;
+    s->pool_peak = s->pool_bytes;
     return AXHEL_HIP_OK;
 }
 

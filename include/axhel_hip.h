@@ -305,6 +305,10 @@ typedef struct axhel_hip_serve_stats {
     double p50_us, p99_us, mean_us, max_us;
     double first_arrival_to_last_completion_s;
     uint64_t descriptor_bytes; /* the slots' device descriptor buffers (charged to the budget) */
+    /* device bytes of the staging pool, blocks in use and cached; cached
+       blocks are freed before an allocation would take the pool plus the
+       descriptor buffers past the budget */
+    uint64_t pool_bytes, peak_pool_bytes;
 } axhel_hip_serve_stats;
 
 typedef struct axhel_hip_server axhel_hip_server;

@@ -210,3 +210,25 @@ def test_latency_record_is_bounded():
     s.reset_stats()
     assert s.stats()["completed"] == 0 and s.stats()["max_us"] == 0
     s.close()
+
+
+def test_staging_pool_stays_within_budget():
+    """Cached staging blocks are freed before a new block would take the pool
+    past the budget: 1 MiB blocks from small requests, then 2 MiB blocks for
+    larger ones, within 8 MiB (without the trimming the pool would hold both
+    sets, 12 MiB)."""
+    s = ax.Server(backend=ax.BACKEND_NULL, null_latency_us=2000, memory_budget_bytes=8 * MIB, slots=2)
+    for limbs in (1, 64, 1, 64):  # 3 buffers of 1 MiB, then of 2 MiB, per request
+        keep, tickets = [], []
+        for _ in range(2):
+            st, t, bufs = req(s, limbs=limbs)
+            assert st == ax.OK
+            tickets.append(t)
+            keep.append(bufs)
+        for t in tickets:
+            assert s.wait(t) == ax.OK
+        stt = s.stats()
+        assert stt["bytes_in_use"] == 0
+        assert stt["pool_bytes"] <= 8 * MIB and stt["peak_pool_bytes"] <= 8 * MIB, stt
+    assert s.stats()["completed"] == 8
+    s.close()
