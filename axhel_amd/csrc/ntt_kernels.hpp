@@ -455,13 +455,63 @@ constexpr bool row_group_in_wave() {
     return (64 % (1 << (LOG_G - LOG_E))) == 0;
 }
 
+// Moves registers through LDS: v[r] is written at word wi(r), then register
+// r is read back from word ri(r) (wi, ri take the compile-time r).  SPLIT
+// moves the words' 32-bit halves in two rounds through an array with the
+// same word indices in 32-bit units: half the bytes, so more workgroups
+// share a CU; the rounds after the first synchronize with group_sync<WAVE>.
+// The synchronization between the writes and the first reads is
+// group_sync<FIRST_WAVE> (a workgroup barrier where it must also publish
+// other LDS data).  The caller orders the writes after earlier reads of the
+// same words.
+template <int E, bool SPLIT, bool WAVE, bool FIRST_WAVE, typename WI, typename RI>
+__device__ __forceinline__ void lds_move(uint64_t (&v)[E], uint64_t *lds, WI wi, RI ri) {
+    if constexpr (SPLIT) {
+        uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
+        uint32_t lo[E];
+        static_for<0, E>([&](auto rc) { l32[wi(rc)] = (uint32_t)v[decltype(rc)::value]; });
+        group_sync<FIRST_WAVE>();
+        static_for<0, E>([&](auto rc) { lo[decltype(rc)::value] = l32[ri(rc)]; });
+        group_sync<WAVE>();
+        static_for<0, E>([&](auto rc) { l32[wi(rc)] = (uint32_t)(v[decltype(rc)::value] >> 32); });
+        group_sync<WAVE>();
+        static_for<0, E>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            v[r] = (uint64_t)lo[r] | ((uint64_t)l32[ri(rc)] << 32);
+        });
+    } else {
+        static_for<0, E>([&](auto rc) { lds[wi(rc)] = v[decltype(rc)::value]; });
+        group_sync<FIRST_WAVE>();
+        static_for<0, E>([&](auto rc) { v[decltype(rc)::value] = lds[ri(rc)]; });
+    }
+}
+
+// The stacked ROW pass moves its words through LDS in 32-bit halves when its
+// rows lie inside one wave (the extra rounds then synchronize one wave).
+template <int LOG_G, int LOG_E>
+constexpr bool stack_split() {
+    return row_group_in_wave<LOG_G, LOG_E>();
+}
+
+// LDS of the stacked ROW pass in 64-bit words: the rows' words (their
+// 32-bit halves when split, rounded up to 16 bytes), then the 2^LOG_G - 1
+// twiddle pairs.
+template <int LOG_G, int LOG_E, int GPW>
+constexpr int stack_lds_words() {
+    constexpr int w = Lds<LOG_G, LOG_E, GPW, false>::WORDS;
+    static_assert(w % 2 == 0, "16-byte aligned twiddle pairs");
+    return (stack_split<LOG_G, LOG_E>() ? (w + 3) / 4 * 2 : w) + 2 * ((1 << LOG_G) - 1);
+}
+
 // The butterfly chunks of one pass over registers v (chunk-0 layout on entry,
 // last-chunk layout on exit), redistributing through LDS between chunks.
 // Every thread of the workgroup must call it (it holds barriers).
 // RLX: forward butterflies on the relaxed schedule (stage_kind).
 // WSYNC: the exchanges synchronize each wave alone (ROW groups inside one
 // wave, row_group_in_wave, whose LDS rows no other wave touches meanwhile).
-template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, bool RLX = false, bool WSYNC = false, typename TW>
+// SPLIT: the exchanges move 32-bit halves (lds_move).
+template <int LOG_G, int LOG_E, int GPW, bool FWD, bool COL, bool RLX = false, bool WSYNC = false,
+          bool SPLIT = false, typename TW>
 __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *lds, uint32_t g, uint32_t tid,
                                            const ModConsts &m, const TW &tws, bool final_inv, const NttJobDev &J) {
     using Gm = Geom<LOG_G, LOG_E, FWD>;
@@ -479,15 +529,9 @@ __device__ __forceinline__ void run_chunks(uint64_t (&v)[1 << LOG_E], uint64_t *
             static_assert(!WSYNC || (!COL && row_group_in_wave<LOG_G, LOG_E>()), "wave-local exchange");
             const uint32_t kp = Gm::kt(c - 1, tid);
             group_sync<WSYNC>();
-            static_for<0, E>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                lds[L::idx2(g, kp, Gm::kr(c - 1, r))] = v[r];
-            });
-            group_sync<WSYNC>();
-            static_for<0, E>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                v[r] = lds[L::idx2(g, ktid, Gm::kr(c, r))];
-            });
+            lds_move<E, SPLIT, WSYNC, WSYNC>(
+                v, lds, [&](auto rc) { return L::idx2(g, kp, Gm::kr(c - 1, decltype(rc)::value)); },
+                [&](auto rc) { return L::idx2(g, ktid, Gm::kr(c, decltype(rc)::value)); });
         }
 
         // butterflies on local bits LO..LO+B-1 (forward: descending)
@@ -753,7 +797,7 @@ ntt_pass_kernel(PassArgs a) {
 // each row needs its own set, twice the row's own bytes, from L2).  Bits,
 // butterflies and twiddles are those of ntt_pass_kernel's ROW pass.
 template <int LOG_G, int LOG_E, int GPW, bool FWD, bool RLX = false>
-__global__ void __launch_bounds__(GPW << (LOG_G - LOG_E))
+__global__ void __launch_bounds__(GPW << (LOG_G - LOG_E)) __attribute__((amdgpu_waves_per_eu(RLX ? 6 : 1)))
 ntt_row_stack_kernel(PassArgs a) {
     using Gm = Geom<LOG_G, LOG_E, FWD>;
     using L = Lds<LOG_G, LOG_E, GPW, false>;
@@ -762,6 +806,11 @@ ntt_row_stack_kernel(PassArgs a) {
     constexpr int T = G / E;
     constexpr int NT = GPW * T;
     constexpr int NC = Gm::NC;
+    // a row's LDS words are its own: when a row lies inside one wave, its
+    // exchanges and its load or store staging synchronize that wave alone,
+    // and the words move in 32-bit halves through half the LDS
+    constexpr bool WS = row_group_in_wave<LOG_G, LOG_E>();
+    constexpr bool SPLIT = stack_split<LOG_G, LOG_E>();
     extern __shared__ uint64_t lds[];
 
     uint32_t tile = blockIdx.x;
@@ -782,43 +831,44 @@ ntt_row_stack_kernel(PassArgs a) {
     const ModConsts m = mod_consts(J0.q);
     const uint32_t n = 1u << a.logn;
 
-    ulonglong2 *tw_lds = reinterpret_cast<ulonglong2 *>(lds + L::WORDS);
+    ulonglong2 *tw_lds = reinterpret_cast<ulonglong2 *>(lds + (stack_lds_words<LOG_G, LOG_E, GPW>() - 2 * (G - 1)));
     TwStage<LOG_G, NT> tstage;
     tstage.template fetch_row<FWD, LOG_E>(J0.tw, n, o, t);
 
+    // an empty slot reads slot 0's row (always a job) and stores nothing:
+    // no separate path for its registers
+    const uint64_t *lsrc = valid ? src : (a.first_pass ? J0.src : J0.dst);
     const uint64_t row = (uint64_t)o << LOG_G;
+    // word (2 tid + (r & 1), (r >> 1) 2T) of the row: register r of the
+    // thread-contiguous 16-byte units in which the inverse loads a row and the
+    // forward stores one (the register part 2T (r >> 1) is disjoint from the
+    // thread part 2 tid + 1)
+    auto unit_word = [&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        return L::idx2(g, 2u * tid + (uint32_t)(r & 1), 2u * (uint32_t)(r >> 1) * T);
+    };
     uint64_t v[E];
     if constexpr (!FWD) {
-        // inverse: chunk 0 is thread-contiguous; copy the thread's own row
-        // through LDS in 16-byte units (T threads x E/2 units per row)
-        if (valid) {
-            if (a.vec16) {
+        // inverse: chunk 0 is thread-contiguous; load the row in 16-byte
+        // units and redistribute them through LDS
+        if (a.vec16) {
 #pragma unroll
-                for (int i = 0; i < E / 2; ++i) {
-                    const uint32_t kr = 2u * (uint32_t)i * T;  // disjoint from the thread part 2 tid (+1)
-                    const ulonglong2 x2 = ld_nt2(src + row + 2u * tid + kr);
-                    lds[L::idx2(g, 2u * tid, kr)] = x2.x;
-                    lds[L::idx2(g, 2u * tid + 1u, kr)] = x2.y;
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < E; ++i) {
-                    const uint32_t kr = (uint32_t)i * T;
-                    lds[L::idx2(g, tid, kr)] = ld_g(src + row + tid + kr);
-                }
+            for (int i = 0; i < E / 2; ++i) {
+                const ulonglong2 x2 = ld_nt2(lsrc + row + 2u * tid + 2u * (uint32_t)i * T);
+                v[2 * i] = x2.x;
+                v[2 * i + 1] = x2.y;
             }
+        } else {
+#pragma unroll
+            for (int i = 0; i < E; ++i)
+                v[i] = ld_g(lsrc + row + 2u * tid + (uint32_t)(i & 1) + 2u * (uint32_t)(i >> 1) * T);
         }
         tstage.store(tw_lds, t);
-        __syncthreads();  // also publishes the twiddles
         const uint32_t k0 = Gm::kt(0, tid);
-        static_for<0, E>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
-            v[r] = lds[L::idx2(g, k0, Gm::kr(0, r))];
-        });
+        // the first barrier is the workgroup's: it also publishes the twiddles
+        lds_move<E, SPLIT, WS, false>(v, lds, unit_word,
+                                      [&](auto rc) { return L::idx2(g, k0, Gm::kr(0, decltype(rc)::value)); });
     } else {
-        // an empty slot reads slot 0's row (always a job) and stores nothing:
-        // no separate path for its registers
-        const uint64_t *lsrc = valid ? src : (a.first_pass ? J0.src : J0.dst);
         const uint32_t k0 = Gm::kt(0, tid);
         static_for<0, E>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
@@ -828,13 +878,10 @@ ntt_row_stack_kernel(PassArgs a) {
         __syncthreads();  // publishes the twiddles
     }
 
-    // a row's LDS words are its own: when a row lies inside one wave, its
-    // exchanges and its store staging synchronize that wave alone
-    constexpr bool WS = row_group_in_wave<LOG_G, LOG_E>();
     // one row o and one plan per workgroup: the top chunk's entries are uniform
     const TwTopChunkUniform<top_chunk<LOG_G, LOG_E, FWD>(), TwGlobal<LOG_G, FWD, false>, TwLdsT<LOG_G, LOG_E, FWD>> tws{
         {J0.tw, n, 0u, o}, {tw_lds}};
-    run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX, WS>(v, lds, g, tid, m, tws, false, J0);
+    run_chunks<LOG_G, LOG_E, GPW, FWD, false, RLX, WS, SPLIT>(v, lds, g, tid, m, tws, false, J0);
 
     if (a.normalize) {
         static_for<0, E>([&](auto rc) {
@@ -846,29 +893,24 @@ ntt_row_stack_kernel(PassArgs a) {
 
     const uint32_t kl = Gm::kt(NC - 1, tid);
     if constexpr (FWD) {
-        // forward: the last chunk is thread-contiguous; store through LDS
+        // forward: the last chunk is thread-contiguous; store the row in
+        // 16-byte units redistributed through LDS
         group_sync<WS>();
-        static_for<0, E>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
-            lds[L::idx2(g, kl, Gm::kr(NC - 1, r))] = v[r];
-        });
-        group_sync<WS>();
+        lds_move<E, SPLIT, WS, WS>(
+            v, lds, [&](auto rc) { return L::idx2(g, kl, Gm::kr(NC - 1, decltype(rc)::value)); }, unit_word);
         if (valid) {
             if (a.vec16) {
 #pragma unroll
                 for (int i = 0; i < E / 2; ++i) {
-                    const uint32_t kr = 2u * (uint32_t)i * T;
                     ulonglong2 x2;
-                    x2.x = lds[L::idx2(g, 2u * tid, kr)];
-                    x2.y = lds[L::idx2(g, 2u * tid + 1u, kr)];
-                    st_g2(dst + row + 2u * tid + kr, x2);
+                    x2.x = v[2 * i];
+                    x2.y = v[2 * i + 1];
+                    st_g2(dst + row + 2u * tid + 2u * (uint32_t)i * T, x2);
                 }
             } else {
 #pragma unroll
-                for (int i = 0; i < E; ++i) {
-                    const uint32_t kr = (uint32_t)i * T;
-                    st_g(dst + row + tid + kr, lds[L::idx2(g, tid, kr)]);
-                }
+                for (int i = 0; i < E; ++i)
+                    st_g(dst + row + 2u * tid + (uint32_t)(i & 1) + 2u * (uint32_t)(i >> 1) * T, v[i]);
             }
         }
     } else if (valid) {

@@ -75,7 +75,7 @@ PassKernel pk_stack() {
     k.multi = false;
     k.stack = true;
     k.threads = GPW << (LG - LE);
-    k.lds_bytes = ((size_t)axhel_dev::Lds<LG, LE, GPW, false>::WORDS + 2 * ((1u << LG) - 1)) * sizeof(uint64_t);
+    k.lds_bytes = (size_t)axhel_dev::stack_lds_words<LG, LE, GPW>() * sizeof(uint64_t);
     return k;
 }
 
