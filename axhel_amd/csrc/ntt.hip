@@ -193,11 +193,12 @@ hipError_t ensure_fn_lds(const void *fn, size_t bytes, int dev) {
 }
 
 hipError_t ensure_lds(const PassKernel &k) {
-    if (k.lds_bytes <= 65536) return hipSuccess;
+    if (k.lds_bytes <= 65536 && k.lds_bytes_relaxed <= 65536) return hipSuccess;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = ensure_fn_lds(k.fn, k.lds_bytes, dev);
-    if (e == hipSuccess && k.fn_relaxed) e = ensure_fn_lds(k.fn_relaxed, k.lds_bytes, dev);
+    if (e == hipSuccess && k.lds_bytes > 65536) e = ensure_fn_lds(k.fn, k.lds_bytes, dev);
+    if (e == hipSuccess && k.fn_relaxed && k.lds_bytes_relaxed > 65536)
+        e = ensure_fn_lds(k.fn_relaxed, k.lds_bytes_relaxed, dev);
     return e;
 }
 
@@ -336,7 +337,7 @@ hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool no
         if (blocks == 0 || blocks * (uint64_t)k.gpw > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
         void *args[] = {&a};
         e = hipLaunchKernel(relaxed ? k.fn_relaxed : k.fn, dim3((unsigned)blocks), dim3(k.threads), args,
-                            k.lds_bytes, stream);
+                            relaxed ? k.lds_bytes_relaxed : k.lds_bytes, stream);
         if (e != hipSuccess) return e;
     }
     return hipGetLastError();

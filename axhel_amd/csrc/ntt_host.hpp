@@ -47,7 +47,8 @@ struct PassKernel {
     bool multi = false;
     bool stack = false;  // ROW pass over stacks of same-plan jobs
     int threads = 0;
-    size_t lds_bytes = 0;
+    size_t lds_bytes = 0;          // fn's dynamic LDS
+    size_t lds_bytes_relaxed = 0;  // fn_relaxed's (its tile layout may differ)
 };
 
 // One transform configuration for a degree: the passes in execution order,

@@ -159,9 +159,8 @@ struct Lds {
     static constexpr int ROW_STRIDE = G + (G >> LOG_E);
     static constexpr int WORDS = COL ? GPW * G : GPW * ROW_STRIDE;
     // COL passes stage the pass's G - 1 twiddle pairs after the data tile
-    // (WORDS is even, so the pairs stay 16-byte aligned).
+    // (pass_tile_words).
     static constexpr int TW_WORDS = COL ? 2 * (G - 1) : 0;
-    static constexpr int TOTAL_WORDS = WORDS + TW_WORDS;
     __device__ static uint32_t idx(uint32_t g, uint32_t k) {
         if constexpr (COL) return k * GPW + g;
         else return g * ROW_STRIDE + k + (k >> LOG_E);
@@ -493,6 +492,24 @@ constexpr bool stack_split() {
     return row_group_in_wave<LOG_G, LOG_E>();
 }
 
+// A COL pass moves its exchanges' words in 32-bit halves (lds_move; its
+// groups span the workgroup, so every round takes a workgroup barrier),
+// except on the relaxed forward schedule, where the extra barriers and LDS
+// instructions cost more than the occupancy gives.
+template <int LOG_G, int LOG_E, bool COL, bool RLX>
+constexpr bool col_split() {
+    return COL && !RLX && Geom<LOG_G, LOG_E, true>::NC > 1;
+}
+
+// Words (64-bit units) of the data tile in the LDS of ntt_pass_kernel: the
+// tile, or its 32-bit halves rounded up to 16 bytes for a split COL pass.
+template <int LOG_G, int LOG_E, int GPW, bool COL, bool RLX>
+constexpr int pass_tile_words() {
+    constexpr int w = Lds<LOG_G, LOG_E, GPW, COL>::WORDS;
+    static_assert(w % 2 == 0, "16-byte aligned twiddle pairs");
+    return col_split<LOG_G, LOG_E, COL, RLX>() ? (w + 3) / 4 * 2 : w;
+}
+
 // LDS of the stacked ROW pass in 64-bit words: the rows' words (their
 // 32-bit halves when split, rounded up to 16 bytes), then the 2^LOG_G - 1
 // twiddle pairs.
@@ -643,7 +660,7 @@ ntt_pass_kernel(PassArgs a) {
     // ---- COL passes: stage the pass's twiddles in LDS (one contiguous range
     // per pass, shared by every column of the limb); the barrier below the
     // data loads publishes them.
-    ulonglong2 *tw_lds = reinterpret_cast<ulonglong2 *>(lds + L::WORDS);
+    ulonglong2 *tw_lds = reinterpret_cast<ulonglong2 *>(lds + pass_tile_words<LOG_G, LOG_E, GPW, COL, RLX>());
     TwStage<LOG_G, NT> tstage;
     if constexpr (COL) tstage.template fetch_col<FWD>(tw, n, t);
 
@@ -713,7 +730,8 @@ ntt_pass_kernel(PassArgs a) {
         // one limb per workgroup: the top chunk's entries are uniform
         const TwTopChunkUniform<top_chunk<LOG_G, LOG_E, FWD>(), TwGlobal<LOG_G, FWD, true>, TwLds<LOG_G, FWD>> tws{
             {tw, n, blo, o}, {tw_lds}};
-        run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX>(v, lds, g, tid, m, tws, a.final_inv != 0, J);
+        run_chunks<LOG_G, LOG_E, GPW, FWD, COL, RLX, false, col_split<LOG_G, LOG_E, COL, RLX>()>(
+            v, lds, g, tid, m, tws, a.final_inv != 0, J);
     } else {
         // a ROW group's exchanges touch its own LDS words only (the staged
         // loads above and the staged stores below are published by

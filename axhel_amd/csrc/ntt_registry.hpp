@@ -27,7 +27,9 @@ PassKernel pk() {
     k.col = COL;
     k.multi = MULTI;
     k.threads = GPW << (LG - LE);
-    k.lds_bytes = (size_t)axhel_dev::Lds<LG, LE, GPW, COL>::TOTAL_WORDS * sizeof(uint64_t);
+    constexpr size_t tw = axhel_dev::Lds<LG, LE, GPW, COL>::TW_WORDS;
+    k.lds_bytes = ((size_t)axhel_dev::pass_tile_words<LG, LE, GPW, COL, false>() + tw) * sizeof(uint64_t);
+    k.lds_bytes_relaxed = ((size_t)axhel_dev::pass_tile_words<LG, LE, GPW, COL, true>() + tw) * sizeof(uint64_t);
     return k;
 }
 
@@ -43,6 +45,7 @@ PassKernel pk_fused() {
     k.multi = false;
     k.threads = GPW << (LG - LE);
     k.lds_bytes = (size_t)axhel_dev::Lds<LG, LE, GPW, false>::WORDS * sizeof(uint64_t);
+    k.lds_bytes_relaxed = k.lds_bytes;
     return k;
 }
 
@@ -59,6 +62,7 @@ PassKernel pk_fused_stack() {
     k.stack = true;
     k.threads = GPW << (LG - LE);
     k.lds_bytes = ((size_t)axhel_dev::Lds<LG, LE, GPW, false>::WORDS + 4 * ((1u << LG) - 1)) * sizeof(uint64_t);
+    k.lds_bytes_relaxed = k.lds_bytes;
     return k;
 }
 
@@ -76,6 +80,7 @@ PassKernel pk_stack() {
     k.stack = true;
     k.threads = GPW << (LG - LE);
     k.lds_bytes = (size_t)axhel_dev::stack_lds_words<LG, LE, GPW>() * sizeof(uint64_t);
+    k.lds_bytes_relaxed = k.lds_bytes;
     return k;
 }
 

@@ -128,7 +128,7 @@ hipError_t issue(axhel_hip_polymul *pm, hipStream_t st) {
             if (blocks == 0 || blocks * (uint64_t)fk.gpw > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
             void *args[] = {&a};
             if ((e = hipLaunchKernel(rlx ? fk.fn_relaxed : fk.fn, dim3((unsigned)blocks), dim3(fk.threads), args,
-                                     fk.lds_bytes, st)))
+                                     rlx ? fk.lds_bytes_relaxed : fk.lds_bytes, st)))
                 return e;
             if (cfg.npass == 2) {
                 if ((e = launch_ntt(cfg, g.logn, true, true, inv, none, g.count, st, 1, g.vec16))) return e;
