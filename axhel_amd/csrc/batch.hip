@@ -32,10 +32,13 @@ struct axhel_hip_batch {
         size_t offset;  // first descriptor in d_jobs
         bool vec16;     // all src/dst 16-byte aligned
         bool relax_ok;  // every modulus below 2^60 (relaxed forward schedule allowed)
-        // stacked ROW pass: stack table at d_jobs + stack_offset (stack_gpw 0: none)
+        // stacked ROW pass: stack table at d_jobs + stack_offset (stack_gpw 0:
+        // none), the jobs outside every stack at d_jobs + rest_offset
         size_t stack_offset;
         uint32_t stacks;
         int stack_gpw;
+        size_t rest_offset;
+        uint32_t rest_count;
     };
     std::vector<Group> groups;
     NttJobDev *d_jobs = nullptr;
@@ -65,6 +68,8 @@ StackTable stack_of(const axhel_hip_batch *b, const axhel_hip_batch::Group &g) {
         t.jobs = b->d_jobs + g.stack_offset;
         t.count = g.stacks;
         t.gpw = g.stack_gpw;
+        t.rest = g.rest_count ? b->d_jobs + g.rest_offset : nullptr;
+        t.rest_count = g.rest_count;
     }
     return t;
 }
@@ -179,19 +184,24 @@ axhel_hip_status axhel_hip_ntt_batch_create(const axhel_hip_ntt_job *jobs, size_
         g.stack_offset = 0;
         g.stacks = 0;
         g.stack_gpw = 0;
+        g.rest_offset = 0;
+        g.rest_count = 0;
         b->groups.push_back(g);
     }
-    // Stack tables (after all plain descriptors) for groups whose ROW pass
-    // can run over same-plan stacks.
+    // Stack tables and the jobs left out of them (after all plain
+    // descriptors) for groups whose ROW pass can run over same-plan stacks.
     for (auto &g : b->groups) {
         const int gpw = stack_gpw_of(config_for_launch(g.logn, g.count));
-        std::vector<NttJobDev> st;
+        std::vector<NttJobDev> st, rest;
         uint32_t stacks = 0;
-        if (gpw && build_stacks(host.data() + g.offset, g.count, gpw, st, stacks)) {
+        if (gpw && build_stacks(host.data() + g.offset, g.count, gpw, st, stacks, rest)) {
             g.stack_offset = host.size();
             g.stacks = stacks;
             g.stack_gpw = gpw;
             host.insert(host.end(), st.begin(), st.end());
+            g.rest_offset = host.size();
+            g.rest_count = (uint32_t)rest.size();
+            host.insert(host.end(), rest.begin(), rest.end());
         }
     }
     // Largest groups first: they fill the device while smaller ones trail.

@@ -249,30 +249,35 @@ int stack_gpw(uint32_t logn) {
     return stack_gpw_of(config_for(logn));
 }
 
-bool build_stacks(const NttJobDev *jobs, uint32_t count, int gpw, std::vector<NttJobDev> &out, uint32_t &stacks) {
+bool build_stacks(const NttJobDev *jobs, uint32_t count, int gpw, std::vector<NttJobDev> &out, uint32_t &stacks,
+                  std::vector<NttJobDev> &rest) {
     out.clear();
+    rest.clear();
     stacks = 0;
     if (gpw <= 0 || count == 0) return false;
-    uint64_t slots = 0;
+    const uint32_t g = (uint32_t)gpw;
     for (uint32_t i = 0; i < count;) {
         uint32_t e = i + 1;
         while (e < count && jobs[e].tw == jobs[i].tw && jobs[e].q == jobs[i].q) ++e;
-        slots += ((uint64_t)(e - i) + gpw - 1) / gpw * gpw;
+        const uint32_t run = e - i, tail = run % g;
+        // a partial stack with at most one empty slot in eight is padded
+        const bool pad = tail && (g - tail) * 8 <= g;
+        const uint32_t stacked = pad ? run : run - tail;
+        for (uint32_t k = i; k < i + stacked; ++k) out.push_back(jobs[k]);
+        if (pad) {
+            NttJobDev empty = jobs[i];
+            empty.src = nullptr;
+            empty.dst = nullptr;
+            while (out.size() % g) out.push_back(empty);
+        }
+        for (uint32_t k = i + stacked; k < e; ++k) rest.push_back(jobs[k]);
         i = e;
     }
-    if ((slots - count) * 8 > slots) return false;
-    out.reserve(slots);
-    for (uint32_t i = 0; i < count;) {
-        uint32_t e = i + 1;
-        while (e < count && jobs[e].tw == jobs[i].tw && jobs[e].q == jobs[i].q) ++e;
-        for (uint32_t k = i; k < e; ++k) out.push_back(jobs[k]);
-        NttJobDev empty = jobs[i];
-        empty.src = nullptr;
-        empty.dst = nullptr;
-        while (out.size() % gpw) out.push_back(empty);
-        i = e;
+    if (out.empty()) {
+        rest.clear();
+        return false;
     }
-    stacks = (uint32_t)(out.size() / gpw);
+    stacks = (uint32_t)(out.size() / g);
     return true;
 }
 
@@ -339,6 +344,22 @@ hipError_t launch_ntt(const NttConfig &cfg, uint32_t logn, bool inverse, bool no
         e = hipLaunchKernel(relaxed ? k.fn_relaxed : k.fn, dim3((unsigned)blocks), dim3(k.threads), args,
                             relaxed ? k.lds_bytes_relaxed : k.lds_bytes, stream);
         if (e != hipSuccess) return e;
+        if (stacked && stacks->rest_count) {
+            // the jobs outside every stack: the per-limb ROW kernel of the same pass
+            const PassKernel &kr = inverse ? cfg.inv[p] : cfg.fwd[p];
+            if (kr.multi) return hipErrorInvalidConfiguration;
+            if ((e = ensure_lds(kr)) != hipSuccess) return e;
+            PassArgs r = a;
+            r.jobs = stacks->rest;
+            r.num_limbs = stacks->rest_count;
+            r.groups_per_limb = (uint32_t)(n >> kr.log_g);
+            const uint64_t rblocks = (uint64_t)stacks->rest_count * r.groups_per_limb / kr.gpw;
+            if (rblocks == 0 || rblocks * (uint64_t)kr.gpw > 0x7FFFFFFFull) return hipErrorInvalidConfiguration;
+            void *rargs[] = {&r};
+            e = hipLaunchKernel(relaxed ? kr.fn_relaxed : kr.fn, dim3((unsigned)rblocks), dim3(kr.threads), rargs,
+                                relaxed ? kr.lds_bytes_relaxed : kr.lds_bytes, stream);
+            if (e != hipSuccess) return e;
+        }
     }
     return hipGetLastError();
 }

@@ -68,18 +68,24 @@ struct NttConfig {
 };
 
 // Jobs of one launch regrouped for the stacked ROW pass: `count` stacks of
-// `gpw` slots, each stack's jobs sharing a plan, empty slots with dst null.
+// `gpw` slots, each stack's jobs sharing a plan, empty slots with dst null;
+// the jobs left out of every stack (`rest`, rest_count of them) run the
+// per-limb ROW kernel beside the stacked one.
 struct StackTable {
     const axhel_dev::NttJobDev *jobs = nullptr;
     uint32_t count = 0;
     int gpw = 0;
+    const axhel_dev::NttJobDev *rest = nullptr;
+    uint32_t rest_count = 0;
 };
 
 // Builds the stacked job table from `jobs` ordered so that jobs of one plan
-// are adjacent (runs of equal table and modulus).  Returns false, leaving
-// `out` empty, when empty slots would exceed one in eight.
+// are adjacent (runs of equal table and modulus): each plan's jobs fill
+// whole stacks, and its last partial stack is padded with empty slots when
+// at most one slot in eight stays empty, else its jobs go to `rest`.
+// Returns false, leaving both outputs empty, when no stack forms.
 bool build_stacks(const axhel_dev::NttJobDev *jobs, uint32_t count, int gpw, std::vector<axhel_dev::NttJobDev> &out,
-                  uint32_t &stacks);
+                  uint32_t &stacks, std::vector<axhel_dev::NttJobDev> &rest);
 
 // Compiled configurations for log2(degree) (empty for unsupported degrees).
 const std::vector<NttConfig> &configs_for(uint32_t logn);

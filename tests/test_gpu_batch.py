@@ -148,13 +148,15 @@ def test_in_place_batch_jobs():
 @pytest.mark.parametrize("logn", [12, 13, 14, 15, 16, 17, 18, 19, 20])
 def test_stacked_row_pass_every_config(logn):
     """Many jobs per plan: the ROW pass runs over stacks of same-plan jobs
-    (one twiddle set per workgroup, a padded last stack per plan).  For every
-    two-pass configuration, both directions and both output ranges, every
-    job equals the plain per-limb call and one job per plan equals the
-    oracle."""
+    (one twiddle set per workgroup).  A plan's last partial stack is padded
+    when at most one slot in eight stays empty (2 gpw - 1 jobs, gpw >= 8);
+    otherwise its jobs (3 gpw + 1: one) run the per-limb ROW kernel beside
+    the stacked one.  For every two-pass configuration, both directions and
+    both output ranges, every job equals the plain per-limb call and the last
+    job of each plan equals the oracle."""
     torch = torch_mod()
     n = 1 << logn
-    qs = params.rns_primes(n, 2)
+    qs = params.rns_primes(n, 3)
     psis = [params.find_primitive_2n_root(n, q) for q in qs]
     plans = [ax.NTT(n, q, p) for q, p in zip(qs, psis)]
     tabs = [O.Tables(n, q, p) for q, p in zip(qs, psis)]
@@ -165,7 +167,7 @@ def test_stacked_row_pass_every_config(logn):
                 continue  # single-pass configurations have no stacked ROW pass
             ax.config_select(n, ci)
             gpw = ax.config_get(n, ci)["row_groups"]
-            counts = [4 * gpw, 3 * gpw + 1]  # one padded stack: 1 empty slot in 8 or fewer
+            counts = [4 * gpw, 3 * gpw + 1, 2 * gpw - 1]  # whole stacks; one job left out; a padded stack
             owner = [pi for pi, c in enumerate(counts) for _ in range(c)]
             for inverse in (False, True):
                 f_in = 2 if inverse else 4
@@ -185,7 +187,7 @@ def test_stacked_row_pass_every_config(logn):
                     assert torch.equal(out, ref), (logn, ci, inverse, lazy)
                     got = host(out).reshape(len(owner), n)
                     for pi in range(len(counts)):
-                        i = owner.index(pi) + counts[pi] - 1  # last job of the plan (in the padded stack)
+                        i = owner.index(pi) + counts[pi] - 1  # last job of the plan
                         want = (O.compute_inverse(host_in[i], tabs[pi], f_in, of) if inverse
                                 else O.compute_forward(host_in[i], tabs[pi], f_in, of))
                         assert np.array_equal(got[i], want), (logn, ci, inverse, lazy, pi)
