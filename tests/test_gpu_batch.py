@@ -196,6 +196,54 @@ def test_stacked_row_pass_every_config(logn):
         ax.config_select(n, -1)
 
 
+@pytest.mark.parametrize("logn", [14, 16, 17])
+def test_stacked_row_pass_on_8_byte_aligned_buffers(logn):
+    """The stacked ROW pass (whole stacks, a padded stack and the per-limb
+    rest) on buffers one word into an allocation, which take its 8-byte
+    load and store paths: every two-pass configuration, both directions and
+    both output ranges, every job equal to the plain call on a 16-byte
+    aligned copy."""
+    torch = torch_mod()
+    n = 1 << logn
+    qs = params.rns_primes(n, 3)
+    plans = [ax.NTT(n, q, params.find_primitive_2n_root(n, q)) for q in qs]
+    rng = np.random.default_rng(100 + logn)
+    try:
+        for ci in range(ax.config_count(n)):
+            if ax.config_get(n, ci)["log_col"] == 0:
+                continue
+            ax.config_select(n, ci)
+            gpw = ax.config_get(n, ci)["row_groups"]
+            counts = [2 * gpw, gpw + 1, 2 * gpw - 1]
+            owner = [pi for pi, c in enumerate(counts) for _ in range(c)]
+            jobs_n = len(owner)
+            for inverse in (False, True):
+                f_in = 2 if inverse else 4
+                host_in = np.stack([params.uniform(rng, n, f_in * qs[pi]) for pi in owner])
+                src = dev(host_in.reshape(-1)).view(jobs_n, n)
+                base = torch.zeros(2 * jobs_n * n + 1, dtype=torch.int64, device="cuda")
+                usrc = base[1:jobs_n * n + 1].view(jobs_n, n)
+                uout = base[jobs_n * n + 1:].view(jobs_n, n)
+                assert usrc.data_ptr() % 16 == 8 and uout.data_ptr() % 16 == 8
+                for lazy in (True, False):
+                    of = f_in if lazy else 1
+                    usrc.copy_(src)
+                    uout.fill_(-1)
+                    b = ax.Batch([(plans[pi], uout[i], usrc[i], inverse, of) for i, pi in enumerate(owner)])
+                    assert b.stacked_groups() == 1, (logn, ci)
+                    b.launch()
+                    ref = torch.full_like(src, -1)
+                    for i, pi in enumerate(owner):
+                        call = plans[pi].ComputeInverse if inverse else plans[pi].ComputeForward
+                        call(ref[i], src[i], f_in, of)
+                    torch.cuda.synchronize()
+                    assert torch.equal(uout, ref), (logn, ci, inverse, lazy)
+                    assert int(base[0].item()) == 0
+                    b.close()
+    finally:
+        ax.config_select(n, -1)
+
+
 def test_sparse_plans_do_not_stack():
     """One job per plan would leave most stack slots empty: the batch keeps
     the per-limb ROW pass."""
