@@ -155,6 +155,7 @@ SIGNATURES = {
     "axhel_hip_ntt_config_select": (_st, [_u64, ctypes.c_int]),
     "axhel_hip_ntt_config_selected": (ctypes.c_int, [_u64]),
     "axhel_hip_ntt_config_single": (ctypes.c_int, [_u64]),
+    "axhel_hip_ntt_config_small": (ctypes.c_int, [_u64]),
     "axhel_hip_polymul_create": (_st, [ctypes.POINTER(PolymulJob), ctypes.c_size_t, ctypes.c_int,
                                        ctypes.POINTER(_p)]),
     "axhel_hip_polymul_launch": (_st, [_p, _p]),
@@ -621,3 +622,8 @@ def config_selected(degree):
 def config_single(degree):
     """Index of the configuration single-limb calls use for this degree."""
     return int(_load().axhel_hip_ntt_config_single(degree))
+
+
+def config_small(degree):
+    """Index of the configuration launches of 2 limbs up to 2^21 words use."""
+    return int(_load().axhel_hip_ntt_config_small(degree))

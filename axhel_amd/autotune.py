@@ -5,10 +5,12 @@ thread, groups per workgroup, pass split) on a synthetic batch sized to fill
 the GPU, forward and inverse, and keeps the fastest for batches; it also
 times single-limb calls (the ComputeForward / ComputeInverse drop-ins, one
 synchronous call at a time), whose latency favours other shapes, and keeps
-the fastest of those for single calls.  The result is a small text table,
-one `log2(N) batch-index single-index` line per degree, that the library
-reads at start-up (`axhel_amd/ntt_tuning_gfx950.txt`, or the file named by
-AXHEL_HIP_TUNING).
+the fastest of those for single calls, and times synchronous launches of a
+few limbs for launches of 2 limbs up to 2^21 words.  The result is a small
+text table, one `log2(N) batch-index single-index [small-index]` line per
+degree (the last only where it differs from the single-call choice), that
+the library reads at start-up (`axhel_amd/ntt_tuning_gfx950.txt`, or the
+file named by AXHEL_HIP_TUNING).
 
 Usage: python -m axhel_amd.autotune [--logn 12 13 ...] [--out PATH] [--words W]
 """
@@ -92,6 +94,37 @@ def tune_single(logn, torch, calls=200):
     return min(results, key=lambda r: r["fwd_plus_inv_us"])["index"], results
 
 
+def tune_small(logn, torch, calls=100):
+    """Latency of a launch of a few limbs per configuration: synchronous
+    batches (forward, then inverse) of half the most limbs that take the
+    small-launch configuration (2^21 words), each limb with its own prime."""
+    n = 1 << logn
+    limbs = max(2, min(64, ((1 << 21) >> logn) // 2))
+    primes = params.rns_primes(n, limbs)
+    plans = [ax.NTT(n, q, params.find_primitive_2n_root(n, q)) for q in primes]
+    src = torch.randint(0, min(primes), (limbs, n), dtype=torch.int64, device="cuda")
+    dst = torch.empty_like(src)
+    results = []
+    for ci in range(ax.config_count(n)):
+        ax.config_select(n, ci)
+        us = 0.0
+        for inverse in (False, True):
+            b = ax.Batch([(plans[i], dst[i], src[i], inverse, 1) for i in range(limbs)])
+            for _ in range(calls // 4):
+                b.launch()
+                torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                b.launch()
+                torch.cuda.synchronize()
+            us += (time.perf_counter() - t0) * 1e6 / calls
+            b.close()
+        results.append({"index": ci, "config": ax.config_get(n, ci), "limbs": limbs, "fwd_plus_inv_us": us})
+        print(json.dumps({"logn": logn, "small": True, **results[-1]}), flush=True)
+    ax.config_select(n, -1)
+    return min(results, key=lambda r: r["fwd_plus_inv_us"])["index"], results
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--logn", type=int, nargs="*", default=list(range(10, 21)))
@@ -109,7 +142,8 @@ def main():
         ax.EltwiseReduceMod(x, x, x.numel(), (1 << 50) - 27, 4, 2)
         torch.cuda.synchronize()
     del x
-    header = "# log2(N) batch-config-index single-call-config-index, chosen by axhel_amd.autotune on gfx950\n"
+    header = ("# log2(N) batch-config-index single-call-config-index [small-launch-config-index], "
+              "chosen by axhel_amd.autotune on gfx950\n")
     table, notes = {}, []
     if os.path.exists(args.out):
         for line in open(args.out):
@@ -117,12 +151,13 @@ def main():
             if line.startswith("#"):
                 if line != header:
                     notes.append(line)  # hand-written notes and overrides are kept
-            elif len(parts) in (2, 3):
+            elif len(parts) in (2, 3, 4):
                 table[int(parts[0])] = [int(x) for x in parts[1:]]
     for logn in args.logn:
         best, _ = tune_degree(logn, torch, args.words, args.reps)
         single, _ = tune_single(logn, torch)
-        table[logn] = [best, single]
+        small, _ = tune_small(logn, torch)
+        table[logn] = [best, single] + ([small] if small != single else [])
     with open(args.out, "w") as f:
         f.write(header)
         f.writelines(notes)

@@ -121,14 +121,18 @@ std::vector<std::vector<NttConfig>> *g_reg = nullptr;
 std::atomic<int> g_selected[kMaxLogN + 1];
 // configuration of single-limb calls (the plain per-call path); -1: g_selected
 std::atomic<int> g_single[kMaxLogN + 1];
+// configuration of launches of 2 limbs up to kLatencyWords words; -1: g_single
+std::atomic<int> g_small[kMaxLogN + 1];
 // the tuning table's choices (written once, before any selection), which
 // config_select(-1) restores
 int g_tuned[kMaxLogN + 1];
 int g_tuned_single[kMaxLogN + 1];
+int g_tuned_small[kMaxLogN + 1];
 
 // Tuned configuration table written by axhel_amd/autotune.py: lines of
-// "log2(N) index [single-call index]" (the index batches use, and optionally
-// the one single-limb calls use, whose latency favours other shapes).  Read
+// "log2(N) index [single-call index [small-launch index]]" (the index
+// batches use, optionally the one single-limb calls use, whose latency
+// favours other shapes, and the one launches of a few limbs use).  Read
 // from $AXHEL_HIP_TUNING, else from
 // ntt_tuning_gfx950.txt in the package directory above this library.
 void load_tuning(const std::vector<std::vector<NttConfig>> &reg) {
@@ -152,11 +156,12 @@ void load_tuning(const std::vector<std::vector<NttConfig>> &reg) {
     char line[256];
     while (std::fgets(line, sizeof(line), f)) {
         if (line[0] == '#') continue;
-        int logn = -1, idx = -1, single = -1;
-        const int got = std::sscanf(line, "%d %d %d", &logn, &idx, &single);
+        int logn = -1, idx = -1, single = -1, small = -1;
+        const int got = std::sscanf(line, "%d %d %d %d", &logn, &idx, &single, &small);
         if (got >= 2 && logn >= 1 && logn <= (int)kMaxLogN && idx >= 0 && idx < (int)reg[logn].size()) {
             g_tuned[logn] = idx;
-            if (got == 3 && single >= 0 && single < (int)reg[logn].size()) g_tuned_single[logn] = single;
+            if (got >= 3 && single >= 0 && single < (int)reg[logn].size()) g_tuned_single[logn] = single;
+            if (got == 4 && small >= 0 && small < (int)reg[logn].size()) g_tuned_small[logn] = small;
         }
     }
     std::fclose(f);
@@ -168,11 +173,13 @@ const std::vector<std::vector<NttConfig>> &registry() {
         for (uint32_t l = 0; l <= kMaxLogN; ++l) {
             g_tuned[l] = 0;
             g_tuned_single[l] = -1;
+            g_tuned_small[l] = -1;
         }
         load_tuning(*g_reg);
         for (uint32_t l = 0; l <= kMaxLogN; ++l) {
             g_selected[l].store(g_tuned[l]);
             g_single[l].store(g_tuned_single[l]);
+            g_small[l].store(g_tuned_small[l]);
         }
     });
     return *g_reg;
@@ -224,8 +231,16 @@ const NttConfig &config_for_single(uint32_t logn) {
     return config_for(logn);
 }
 
+const NttConfig &config_for_small(uint32_t logn) {
+    const auto &v = configs_for(logn);
+    const int i = g_small[logn].load();
+    if (i >= 0 && i < (int)v.size()) return v[i];
+    return config_for_single(logn);
+}
+
 const NttConfig &config_for_launch(uint32_t logn, uint64_t count) {
-    return count <= (kLatencyWords >> logn) ? config_for_single(logn) : config_for(logn);
+    if (count <= 1) return config_for_single(logn);
+    return count <= (kLatencyWords >> logn) ? config_for_small(logn) : config_for(logn);
 }
 
 hipError_t prepare_config(const NttConfig &cfg) {
@@ -580,9 +595,11 @@ axhel_hip_status axhel_hip_ntt_config_select(uint64_t degree, int index) {
     if (index < 0) {  // -1: back to the tuning table's choices (the first configuration without one)
         g_selected[logn].store(g_tuned[logn]);
         g_single[logn].store(g_tuned_single[logn]);
+        g_small[logn].store(g_tuned_small[logn]);
     } else {
         g_selected[logn].store(index);
-        g_single[logn].store(-1);  // single calls follow an explicit selection
+        g_single[logn].store(-1);  // single calls and small launches follow an explicit selection
+        g_small[logn].store(-1);
     }
     return AXHEL_HIP_OK;
 }
@@ -600,6 +617,14 @@ int axhel_hip_ntt_config_single(uint64_t degree) {
     (void)registry();
     const int i = g_single[logn].load();
     return i >= 0 ? i : g_selected[logn].load();
+}
+
+int axhel_hip_ntt_config_small(uint64_t degree) {
+    uint32_t logn = 0;
+    if (!valid_degree(degree, &logn)) return -1;
+    (void)registry();
+    const int i = g_small[logn].load();
+    return i >= 0 ? i : axhel_hip_ntt_config_single(degree);
 }
 
 }  // extern "C"

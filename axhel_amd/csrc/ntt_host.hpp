@@ -94,14 +94,19 @@ const NttConfig &config_for(uint32_t logn);
 // Configuration of single-limb calls: the tuning table's latency choice, or
 // config_for's once a configuration has been selected explicitly.
 const NttConfig &config_for_single(uint32_t logn);
-// Configuration of one launch over `count` limbs.  Up to kLatencyWords words
-// a launch fills too little of the GPU to be throughput-bound and runs the
-// single-call configuration (measured faster up to 32 limbs at 2^16 and 64+
-// at 2^13 and 2^14); larger launches run the batch configuration.
+// Configuration of launches of 2 limbs up to kLatencyWords words: the
+// tuning table's small-launch entry, else config_for_single's.
+const NttConfig &config_for_small(uint32_t logn);
+// Configuration of one launch over `count` limbs.  One limb runs the
+// single-call configuration; up to kLatencyWords words a launch fills too
+// little of the GPU to be throughput-bound and runs the small-launch
+// configuration (latency shapes, measured faster up to 32 limbs at 2^16 and
+// 64+ at 2^13 and 2^14); larger launches run the batch configuration.
 constexpr uint64_t kLatencyWords = 1ull << 21;
 const NttConfig &config_for_launch(uint32_t logn, uint64_t count);
-// Group count per stack of a configuration's stacked ROW pass (0: none).
-inline int stack_gpw_of(const NttConfig &c) { return c.fwd_stack.fn ? c.fwd_stack.gpw : 0; }
+// Group count per stack of a configuration's stacked ROW pass (0: none; a
+// one-row stack would share its twiddles with nothing).
+inline int stack_gpw_of(const NttConfig &c) { return c.fwd_stack.fn && c.fwd_stack.gpw > 1 ? c.fwd_stack.gpw : 0; }
 
 // Raises per-kernel limits (dynamic LDS above 64 KiB) for every pass of a
 // configuration; done before graph capture so capture records launches only.

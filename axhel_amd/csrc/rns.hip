@@ -325,6 +325,7 @@ axhel_hip_status axhel_hip_rns_create(uint64_t degree, const uint64_t *moduli, c
         NttJobDev *scr = nullptr;
         if (e == hipSuccess) e = scratch_for_device(&scr);
         if (e == hipSuccess) e = prepare_config(config_for_single(c->logn));
+        if (e == hipSuccess) e = prepare_config(config_for_small(c->logn));
         if (e == hipSuccess) e = prepare_config(config_for(c->logn));
         if (e != hipSuccess) {
             set_last_error("rns create", e);

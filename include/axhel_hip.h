@@ -433,6 +433,11 @@ int axhel_hip_ntt_config_selected(uint64_t degree);
    single-call entry for the degree (a shape chosen for latency), or the
    selected configuration once one has been selected. */
 int axhel_hip_ntt_config_single(uint64_t degree);
+/* Index of the configuration launches of 2 limbs up to 2^21 words (batch
+   groups, RNS-context calls, serving groups) use: the tuning table's
+   small-launch entry for the degree, else the single-call configuration.  No
+   reference counterpart. */
+int axhel_hip_ntt_config_small(uint64_t degree);
 
 #ifdef __cplusplus
 }

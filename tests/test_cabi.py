@@ -92,29 +92,35 @@ def test_rns_create_rejects_bad_arguments():
 
 
 def test_tuning_table_sets_batch_and_single_call_configurations():
-    """The shipped table's two columns are what the library reports; an
-    explicit selection applies to single calls too, and -1 restores the
-    table's choices.  Host-side state only, no compute."""
+    """The shipped table's columns (batch, single call, and optionally small
+    launches, which default to the single-call entry) are what the library
+    reports; an explicit selection applies to single calls and small
+    launches too, and -1 restores the table's choices.  Host-side state
+    only, no compute."""
     path = os.path.join(ROOT, "axhel_amd", "ntt_tuning_gfx950.txt")
     table = {}
     for line in open(path):
         parts = line.split()
         if parts and not line.startswith("#"):
             table[int(parts[0])] = [int(x) for x in parts[1:]]
-    assert all(len(v) == 2 for v in table.values())
-    for logn, (batch, single) in table.items():
+    assert all(len(v) in (2, 3) for v in table.values())
+    for logn, (batch, single, *small) in table.items():
         n = 1 << logn
         assert axhel_amd.config_selected(n) == batch and axhel_amd.config_single(n) == single, logn
+        assert axhel_amd.config_small(n) == (small[0] if small else single), logn
     n = 1 << 16
     try:
         axhel_amd.config_select(n, 2)
         assert axhel_amd.config_selected(n) == 2 and axhel_amd.config_single(n) == 2
+        assert axhel_amd.config_small(n) == 2
     finally:
         axhel_amd.config_select(n, -1)
     assert axhel_amd.config_selected(n) == table[16][0] and axhel_amd.config_single(n) == table[16][1]
-    # a degree the table does not list: the first configuration for both
+    assert axhel_amd.config_small(n) == table[16][-1]
+    # a degree the table does not list: the first configuration for all three
     assert 9 not in table
     assert axhel_amd.config_selected(1 << 9) == 0 and axhel_amd.config_single(1 << 9) == 0
+    assert axhel_amd.config_small(1 << 9) == 0
 
 
 def test_missing_library_fails_loudly():

@@ -163,8 +163,8 @@ def test_stacked_row_pass_every_config(logn):
     rng = np.random.default_rng(logn)
     try:
         for ci in range(ax.config_count(n)):
-            if ax.config_get(n, ci)["log_col"] == 0:
-                continue  # single-pass configurations have no stacked ROW pass
+            if ax.config_get(n, ci)["log_col"] == 0 or ax.config_get(n, ci)["row_groups"] < 2:
+                continue  # no stacked ROW pass: single-pass, or one row per workgroup (nothing to share)
             ax.config_select(n, ci)
             gpw = ax.config_get(n, ci)["row_groups"]
             counts = [4 * gpw, 3 * gpw + 1, 2 * gpw - 1]  # whole stacks; one job left out; a padded stack
@@ -210,7 +210,7 @@ def test_stacked_row_pass_on_8_byte_aligned_buffers(logn):
     rng = np.random.default_rng(100 + logn)
     try:
         for ci in range(ax.config_count(n)):
-            if ax.config_get(n, ci)["log_col"] == 0:
+            if ax.config_get(n, ci)["log_col"] == 0 or ax.config_get(n, ci)["row_groups"] < 2:
                 continue
             ax.config_select(n, ci)
             gpw = ax.config_get(n, ci)["row_groups"]
